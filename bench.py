@@ -1,0 +1,159 @@
+"""Throughput of the MI355X SPH step (one JSON line on stdout).
+
+Default workload: 1,048,576 particles (1024x1024 block at 0.5h, seeded
+jitter) in a 600h x 600h box with 4 walls and 64 circles; h=1, rho0=10,
+k=0.004, k_near=0.01, sigma=0, beta=0.3, g=(0,-9.8), dt=0.005.  A step is one
+full SPH step of that scene.  W warm-up steps run untimed; then K steps are
+timed with the inputs resident in HBM.
+
+roofline: the dominant kernel's algorithmic bytes per launch divided by its
+average duration, measured with HIP events on the library's stream over the
+timed steps.  cpu_baseline: the reference's multi-threaded step restated in
+C++ (oracle/, Gauss-Seidel order, pooled phases), timed on this host on a
+bounded sample that starts from the GPU state after warm-up.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s @1M particles: plain vs surrogate at ≤1% density error"
+HBM_PEAK_GBS = 8000.0
+DT = 0.005
+
+# Algorithmic bytes per particle of each kernel group: every particle record
+# crosses HBM once per phase, neighbour re-reads are on-chip.
+#   viscosity_predict: R x,v (16) + W xpred (8) + W key, fill (8) + R key (4)  = 36
+#   grid:              R key, fill (8) + W slot, id (8) + R slot, id (8)
+#                      + R xpred, x (16) + W x, prev, id, key (24)              = 64
+#   density:           R x, key (12) + W density, pressure (16)                = 28
+#   delta:             R x, key, pressure, prev (28) + W x, v (16)             = 44
+KERNEL_BYTES = {"viscosity_predict": 36, "grid": 64, "density": 28, "delta": 44}
+STEP_ALG_BYTES = 172  # reference phases, one SoA FP32 pass each
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(name):
+    from nbodysimulation_experiment_amd import scenes
+    if name == "cfg1":
+        return [scenes.dam_break_1m()], "dam break, 1,048,576 particles (1024x1024 at 0.5h), 600h x 600h box, 4 walls + 64 circles"
+    if name == "cfg0":
+        return [scenes.dam_break_4k()], "dam break, 4,096 particles (64x64 at 0.5h), 80h x 60h box, 4 walls"
+    if name == "cfg2":
+        sp = scenes.batched_scenes()
+        return sp, f"{len(sp)} independent scenes, {sum(s.n for s in sp):,} particles, counts log-uniform in [2048, 65536]"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(state, specs, threads, sample_steps, warm_steps=1):
+    """Times the oracle's multi-threaded reference-order step on a sample that
+    starts from the GPU state after warm-up."""
+    from nbodysimulation_experiment_amd.scenes import SPHParams
+    from oracle.oracle import GS_MT, OracleSim
+    o = OracleSim(SPHParams(), 32, GS_MT, threads)
+    for s in specs:
+        o.add_scene_spec(s)
+    o.set_gravity(0.0, -9.8)
+    o.set_state(state["x"], state["v"])
+    o.step(DT, warm_steps)  # rebuilds the neighbour lists
+    t0 = time.perf_counter()
+    o.step(DT, sample_steps)
+    t = time.perf_counter() - t0
+    n = o.n
+    o.close()
+    return n * sample_steps / t, t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2"])
+    ap.add_argument("--graph", action="store_true", help="replay steps as HIP graphs (no per-kernel timing)")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build computes on one GPU; runs on more than one GPU are not supported", file=sys.stderr)
+        sys.exit(2)
+
+    from nbodysimulation_experiment_amd.sim import ParticleSimulation
+    from nbodysimulation_experiment_amd.scenes import SPHParams
+
+    specs, desc = build_workload(args.workload)
+    n = sum(s.n for s in specs)
+    log(f"workload {args.workload}: {desc}")
+    sim = ParticleSimulation.from_scene(specs[0], SPHParams()) if len(specs) == 1 else \
+        ParticleSimulation.from_scenes(specs, SPHParams())
+    sim.set_options(hip_graph=args.graph)
+    log(f"warm-up {args.warmup} steps")
+    sim.Update(DT, args.warmup)
+    state_after_warmup = sim.get_state() if not args.no_cpu_baseline else None
+
+    sim.set_timing(not args.graph)
+    sim.ResetStats()
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    sim.Update(DT, args.steps)  # returns after the stream has drained
+    elapsed = time.perf_counter() - t0
+    value = n * args.steps / elapsed
+    kt = sim.kernel_times()
+    stats = sim.GetStats()
+    diag = sim.diagnostics()
+    log(f"{value:.4g} particle-steps/s, {1e3 * elapsed / args.steps:.4f} ms/step; kernels {kt}")
+
+    roofline = None
+    if not args.graph:
+        dom = max(kt, key=lambda k: kt[k][0])
+        avg_ms = kt[dom][0]
+        alg = KERNEL_BYTES[dom] * n
+        achieved = alg / (avg_ms * 1e-3) / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tpath):
+            t = json.load(open(tpath))
+            if t.get("workload") == args.workload and dom in t.get("kernels", {}):
+                traffic = t["kernels"][dom]
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
+                    "kernel_avg_ms": round(avg_ms, 5), "alg_bytes_per_launch": alg,
+                    "step_alg_bytes_frac": round(value * STEP_ALG_BYTES / 1e9 / HBM_PEAK_GBS, 5)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1)
+        log(f"cpu baseline: {threads} threads, {args.cpu_steps} steps")
+        cv, ct = cpu_baseline(state_after_warmup, specs, threads, args.cpu_steps)
+        cpu = {"value": round(cv, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the same scene from the GPU state after {args.warmup} "
+                         f"warm-up steps (+1 untimed step to rebuild lists), {ct:.1f} s",
+               "gpu_over_cpu": round(value / cv, 1)}
+        log(f"cpu baseline {cv:.4g} particle-steps/s")
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": desc, "particles": n, "dt": DT, "method": "plain", "hip_graph": args.graph},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "stats": {k: v for k, v in stats.items() if k != "time"},
+        "diagnostics_fp64": {k: diag[k] for k in ("mean_density", "max_density", "mean_abs_density_error",
+                                                  "kinetic_energy", "non_finite_count")},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,709 @@
+// HIP kernels of the SPH step for gfx950 (MI355X).
+//
+// One step is five launches on one stream:
+//   1. k_visc_predict   integrate, viscosity (Jacobi gather over the previous
+//                       binning), predict, cell key of the predicted position and
+//                       the per-cell arrival count (demo4.cpp:301-356)
+//   2. k_scan_*         exclusive scan of the per-cell counts -> cell starts
+//   3. k_scatter + k_rank_reorder
+//                       bucket the particles by cell, order each cell by particle
+//                       index, and move the state into the new slot order
+//   4. k_density        density, near density, pressure (demo4.cpp:208-221)
+//   5. k_delta          double-density relaxation (Jacobi gather), plane /
+//                       circle / segment / polygon collisions, velocity
+//                       (demo4.cpp:239-255, 411-450)
+//
+// Every FP32 expression keeps the reference's operand order and the library
+// is compiled with -ffp-contract=off and correctly rounded sqrt / divide, so the
+// arithmetic of each pair term is the reference's (sph.h:465-542).
+#include "nbs_kernels.hpp"
+
+namespace nbs {
+
+// ---------------------------------------------------------------------------
+// Helpers
+// ---------------------------------------------------------------------------
+
+// Workgroup index with consecutive tiles kept on one XCD.  The dispatcher
+// places workgroup b on XCD (b mod 8); this remap gives XCD x a contiguous
+// range of tiles so neighbouring cells share that XCD's L2.
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned b = blockIdx.x, nb = gridDim.x;
+  const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u, loc = b >> 3;
+  return (x < r) ? x * (q + 1u) + loc : r * (q + 1u) + (x - r) * q + loc;
+}
+
+// sph.h:450-463 per axis.  Out-of-range values are clamped in floating point so
+// NaN and values beyond the int range are defined (NaN -> 0).
+__device__ __forceinline__ int cell_coord(float p, float half, float cs, int n) {
+  const float f = (p + half) / cs;
+  if (!(f >= 0.0f)) return 0;
+  if (f >= (float)n) return n - 1;
+  return (int)f;
+}
+
+__device__ __forceinline__ int scene_of_key(const uint32_t *base, int n, uint32_t key) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (base[mid] <= key) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool MULTI>
+__device__ __forceinline__ DevScene scene_of_slot(const Buffers &b, const DevScene &s0, uint32_t key) {
+  if (!MULTI) return s0;
+  return b.scenes[scene_of_key(b.sceneBase, b.nScenes, key)];
+}
+
+// ---------------------------------------------------------------------------
+// Boundary collisions (sph.h:514-681), bodies of the scene in insertion order
+// (demo4.cpp:414-441).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void normalize2(float vx, float vy, float &nx, float &ny) {
+  float l = sqrtf(vx * vx + vy * vy);  // vecmath.h:267-280
+  if (l == 0.0f) l = 1.0f;
+  const float inv = 1.0f / l;
+  nx = vx * inv;
+  ny = vy * inv;
+}
+
+__device__ __forceinline__ void solve_plane(const DevParams &p, float &x, float &y, float nx, float ny, float d) {
+  const float ppx = nx * d, ppy = ny * d;                 // sph.h:515
+  const float dx = x - ppx, dy = y - ppy;                 // sph.h:517
+  const float proj = dx * nx + dy * ny;                   // sph.h:518
+  if (proj <= p.R) {                                      // sph.h:519
+    const float pen = p.R - proj;                         // sph.h:520
+    x = (nx * pen) + x;                                   // sph.h:521
+    y = (ny * pen) + y;
+  }
+}
+
+__device__ __forceinline__ void solve_circle(const DevParams &p, float &x, float &y, float cx, float cy, float r) {
+  const float both = r + p.R;                             // sph.h:527
+  const float dx = x - cx, dy = y - cy;                   // sph.h:529
+  const float d2 = dx * dx + dy * dy;                     // sph.h:530
+  if (d2 <= both * both) {                                // sph.h:531
+    if (fabsf(d2) > 0.0f) {                               // sph.h:534
+      const float dist = sqrtf(d2);
+      const float inv = 1.0f / dist;
+      const float nx = dx * inv, ny = dy * inv;           // sph.h:536
+      const float pen = both - dist;                      // sph.h:537
+      x = (nx * pen) + x;                                 // sph.h:538
+      y = (ny * pen) + y;
+    }
+  }
+}
+
+__device__ __forceinline__ float dot2(float ax, float ay, float bx, float by) { return ax * bx + ay * by; }
+
+__device__ __noinline__ void solve_segment(const DevParams &p, float &x, float &y, float ax, float ay, float bx,
+                                           float by) {
+  const float both = p.margin + p.R;                      // sph.h:545
+  const float ex = bx - ax, ey = by - ay;                 // sph.h:548
+  const float u = dot2(ex, ey, bx - x, by - y);           // sph.h:549
+  const float v = dot2(ex, ey, x - ax, y - ay);           // sph.h:550
+  float clx, cly, nx, ny;
+  if (v <= 0.0f) {                                        // sph.h:555-563
+    clx = ax; cly = ay;
+    const float dx = x - clx, dy = y - cly;
+    if (dot2(dx, dy, dx, dy) > both * both) return;
+    normalize2(x - clx, y - cly, nx, ny);
+  } else if (u <= 0.0f) {                                 // sph.h:564-572
+    clx = bx; cly = by;
+    const float dx = x - clx, dy = y - cly;
+    if (dot2(dx, dy, dx, dy) > both * both) return;
+    normalize2(x - clx, y - cly, nx, ny);
+  } else {                                                // sph.h:573-591
+    const float den = dot2(ex, ey, ex, ey);
+    const float inv = 1.0f / den;
+    clx = ((ax * u) + (bx * v)) * inv;
+    cly = ((ay * u) + (by * v)) * inv;
+    const float dx = x - clx, dy = y - cly;
+    if (dot2(dx, dy, dx, dy) > both * both) return;
+    float px = -ey, py = ex;
+    if (dot2(px, py, x - ax, y - ay) < 0.0f) { px = -px; py = -py; }
+    normalize2(px, py, nx, ny);
+  }
+  const float dist = dot2(nx, ny, x - clx, y - cly);      // sph.h:593-594
+  const float pen = both - dist;                          // sph.h:595
+  x = (nx * pen) + x;                                     // sph.h:596
+  y = (ny * pen) + y;
+}
+
+__device__ __noinline__ void solve_polygon(const DevParams &p, float &x, float &y, int nv, const float2 *verts) {
+  // sph.h:600-672 FindMTVCirclePolygon, sph.h:674-681 SPHSolvePolygonCollision
+  const float radius = p.margin + p.R;
+  int edge = 0;
+  float nX = 0.0f, nY = 0.0f, sep = -3.402823466e+38f;
+  for (int k = 0; k < nv; ++k) {
+    const float2 a = verts[k];
+    const float2 bb = verts[(k + 1) % nv];
+    const float ex = bb.x - a.x, ey = bb.y - a.y;
+    float cx, cy;
+    normalize2(1.0f * ey, -1.0f * ex, cx, cy);  // vecmath.h:299-301 Vec2Cross(v, 1)
+    const float s = dot2(cx, cy, x - a.x, y - a.y);
+    if (s > radius) return;
+    if (s > sep) { nX = cx; nY = cy; sep = s; edge = k; }
+  }
+  const float2 v1 = verts[edge];
+  const float2 v2 = verts[(edge + 1) % nv];
+  float mx, my;
+  if (sep < 1.19209290e-07f) {                            // sph.h:626 FLT_EPSILON
+    const float pen = radius - sep;
+    mx = nX * pen; my = nY * pen;
+  } else {
+    const float u1 = dot2(x - v1.x, y - v1.y, v2.x - v1.x, v2.y - v1.y);
+    const float u2 = dot2(x - v2.x, y - v2.y, v1.x - v2.x, v1.y - v2.y);
+    if (u1 <= 0.0f || u2 <= 0.0f) {
+      const float2 c = (u1 <= 0.0f) ? v1 : v2;
+      // vecmath.h:292-296: the reference's "distance squared" is ((bx-ax)*(by-ay))^2
+      const float f = (c.x - x) * (c.y - y);
+      if (f * f > radius * radius) return;
+      const float dx = x - c.x, dy = y - c.y;
+      float ux, uy;
+      normalize2(dx, dy, ux, uy);
+      const float pen = radius - dot2(ux, uy, dx, dy);
+      mx = ux * pen; my = uy * pen;
+    } else {
+      // vecmath.h:228-231 ScalarLerp(a, 0.5, b) = (1 - 0.5) * a + 0.5 * b
+      const float fx = (1.0f - 0.5f) * v1.x + 0.5f * v2.x;
+      const float fy = (1.0f - 0.5f) * v1.y + 0.5f * v2.y;
+      const float s = dot2(x - fx, y - fy, nX, nY);
+      if (s > radius) return;
+      const float pen = radius - s;
+      mx = nX * pen; my = nY * pen;
+    }
+  }
+  x = x + mx;
+  y = y + my;
+}
+
+__device__ __forceinline__ void collide(const Buffers &b, const DevParams &p, const DevScene &sc, float &x, float &y) {
+  for (int k = sc.bodyBegin; k < sc.bodyEnd; ++k) {
+    const DevBody bd = b.bodies[k];
+    switch (bd.type) {
+      case BODY_PLANE: solve_plane(p, x, y, bd.a0, bd.a1, bd.a2); break;
+      case BODY_CIRCLE: solve_circle(p, x, y, bd.a0, bd.a1, bd.a2); break;
+      case BODY_SEGMENT: solve_segment(p, x, y, bd.a0, bd.a1, bd.a2, bd.a3); break;
+      case BODY_POLYGON: solve_polygon(p, x, y, bd.nverts, b.verts + bd.vertOffset); break;
+      default: break;
+    }
+  }
+}
+
+// Candidate ranges of a particle in cell (cx, cy): the 3x3 cells dy = -1..1
+// (outer), dx = -1..1 (inner), restricted to the grid (demo4.cpp:188-203).  The
+// three cells of one row have consecutive keys, so each row is one contiguous
+// range of sorted slots.
+struct RowRange {
+  uint32_t j0, j1;
+};
+
+__device__ __forceinline__ RowRange row_range(const uint32_t *start, const DevScene &sc, int cx, int cy, int dy) {
+  const int yy = cy + dy;
+  if (yy < 0 || yy >= sc.ny) return {0u, 0u};
+  const int xlo = max(cx - 1, 0), xhi = min(cx + 1, sc.nx - 1);
+  const uint32_t row = sc.cellBase + (uint32_t)(yy * sc.nx);
+  return {start[row + xlo], start[row + xhi + 1]};
+}
+
+// ---------------------------------------------------------------------------
+// 1. integrate + viscosity + predict + key + count
+// ---------------------------------------------------------------------------
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_visc_predict(Buffers b, DevParams p, DevScene s0, int n, int nBinned,
+                                                         int lists) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    b.stats[0] = 0xFFFFFFFFu;
+    b.stats[1] = 0u;
+    b.stats[2] = 0xFFFFFFFFu;
+    b.stats[3] = 0u;
+  }
+  const int i = (int)(xcd_block() * kBlock + threadIdx.x);
+  if (i >= n) return;
+  const float2 xi = b.x0[i];
+  const float2 vi = b.v[i];
+  const bool binned = i < nBinned;
+  float aX, aY;
+  DevScene sc;
+  if (binned) {
+    aX = p.gex + 0.0f;                                   // demo4.cpp:306 (own acceleration is zero)
+    aY = p.gey + 0.0f;
+    sc = scene_of_slot<MULTI>(b, s0, b.keys[i]);
+  } else {
+    const float2 a = b.acc[i];
+    aX = p.gex + a.x;
+    aY = p.gey + a.y;
+    sc = MULTI ? b.scenes[b.scn[i]] : s0;
+  }
+  float vx = (aX * p.dt) + vi.x;                         // demo4.cpp:307
+  float vy = (aY * p.dt) + vi.y;
+
+  if (lists && binned) {
+    // demo4.cpp:223-237 in Jacobi order: both halves of each pair's impulse,
+    // computed from the integrated velocities of the previous binning's slots.
+    const uint32_t local = b.keys[i] - sc.cellBase;
+    const int cy = (int)(local / (uint32_t)sc.nx);
+    const int cx = (int)local - cy * sc.nx;
+    float accx = 0.0f, accy = 0.0f;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const RowRange rr = row_range(b.startCur, sc, cx, cy, dy);
+      for (uint32_t j = rr.j0; j < rr.j1; ++j) {
+        const float2 xj = b.x0[j];
+        const float2 vj = b.v[j];
+        const float vjx = p.gdtx + vj.x, vjy = p.gdty + vj.y;
+        const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;  // sph.h:500
+        const float r2 = Rx * Rx + Ry * Ry;               // sph.h:501
+        if (r2 < p.h2) {                                  // sph.h:502
+          const float r = sqrtf(r2);                      // sph.h:503
+          const float q = r * p.invH;                     // sph.h:504
+          const float l = (r == 0.0f) ? 1.0f : r;         // sph.h:505 Vec2Normalize
+          const float inv = 1.0f / l;
+          const float nX = Rx * inv, nY = Ry * inv;
+          const float u = (vx - vjx) * nX + (vy - vjy) * nY;  // sph.h:506
+          if (u > 0.0f) {
+            const float f = (1.0f - q) * (p.sigma * u + p.beta * (u * u));  // sph.h:508
+            const float ix = ((f * nX) * 0.5f) * p.dt;   // demo4.cpp:233 force * 0.5f * deltaTime
+            const float iy = ((f * nY) * 0.5f) * p.dt;
+            accx = accx - ix;                             // own visit
+            accy = accy - iy;
+            accx = accx - ix;                             // neighbour's visit (f_ji = -f_ij)
+            accy = accy - iy;
+          }
+        }
+      }
+    }
+    vx = accx + vx;
+    vy = accy + vy;
+  }
+
+  const float xpx = (vx * p.dt) + xi.x;                  // demo4.cpp:334-335
+  const float xpy = (vy * p.dt) + xi.y;
+  b.xp[i] = make_float2(xpx, xpy);
+  const int cx = cell_coord(xpx, sc.halfW, sc.cellSize, sc.nx);
+  const int cy = cell_coord(xpy, sc.halfH, sc.cellSize, sc.ny);
+  const uint32_t key = sc.cellBase + (uint32_t)(cy * sc.nx + cx);
+  b.keyp[i] = key;
+  b.fill[i] = atomicAdd(&b.count[key], 1u);
+}
+
+// Restores the per-slot scene and zero acceleration of particles that lose
+// their binning (a new scene changes the cell tables; a restored state starts
+// without neighbour lists).
+__global__ __launch_bounds__(kBlock) void k_unbin(Buffers b, int nBinned) {
+  const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (i >= nBinned) return;
+  b.scn[i] = (uint32_t)scene_of_key(b.sceneBase, b.nScenes, b.keys[i]);
+  b.acc[i] = make_float2(0.0f, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// 2. exclusive scan of the per-cell counts
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = (uint32_t)__shfl_up((int)v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread across the workgroup.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total) {
+  __shared__ uint32_t waveTot[kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t incl = wave_incl_scan(v);
+  if (lane == 63) waveTot[wave] = incl;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    const uint32_t t = waveTot[w];
+    if (w < wave) off += t;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + incl - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(Buffers b, uint32_t cells) {
+  const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanItems;
+  uint32_t s = 0, mx = 0, mn = 0xFFFFFFFFu;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint64_t c = base + k;
+    const uint32_t v = (c < cells) ? b.count[c] : 0u;
+    s += v;
+    mx = max(mx, v);
+    if (v > 0u) mn = min(mn, v);
+  }
+  uint32_t tot;
+  block_excl_scan(s, &tot);
+  // occupied-cell min / max
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
+    mn = min(mn, (uint32_t)__shfl_xor((int)mn, d, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (mx > 0u) atomicMax(&b.stats[3], mx);
+    if (mn != 0xFFFFFFFFu) atomicMin(&b.stats[2], mn);
+  }
+  if (threadIdx.x == 0) b.blockSum[blockIdx.x] = tot;
+}
+
+// One workgroup: exclusive scan of the workgroup sums.
+__global__ __launch_bounds__(kBlock) void k_scan_blocks(Buffers b, uint32_t nblocks) {
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < nblocks; base += kBlock) {
+    const uint32_t k = base + threadIdx.x;
+    const uint32_t v = (k < nblocks) ? b.blockSum[k] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan(v, &tot);
+    if (k < nblocks) b.blockOff[k] = carry + ex;
+    carry += tot;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_final(Buffers b, uint32_t cells) {
+  const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint64_t c = base + k;
+    v[k] = (c < cells) ? b.count[c] : 0u;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = b.blockOff[blockIdx.x] + block_excl_scan(s, &tot);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const uint64_t c = base + k;
+    if (c < cells) {
+      b.startNext[c] = run;
+      b.count[c] = 0u;  // ready for the next step's counting
+      run += v[k];
+      if (c == (uint64_t)cells - 1) b.startNext[cells] = run;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. bucket by cell, order each cell by particle index, move the state
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_scatter(Buffers b, int n) {
+  const int s = (int)(xcd_block() * kBlock + threadIdx.x);
+  if (s >= n) return;
+  const uint32_t key = b.keyp[s];
+  const uint32_t pos = b.startNext[key] + b.fill[s];
+  b.tmpSlot[pos] = (uint32_t)s;
+  b.tmpId[pos] = b.id[s];
+}
+
+__global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
+  const int p = (int)(xcd_block() * kBlock + threadIdx.x);
+  if (p >= n) return;
+  const uint32_t s = b.tmpSlot[p];
+  const uint32_t myId = b.tmpId[p];
+  const uint32_t key = b.keyp[s];
+  const uint32_t c0 = b.startNext[key], c1 = b.startNext[key + 1];
+  uint32_t rank = 0;
+  for (uint32_t q = c0; q < c1; ++q) rank += (b.tmpId[q] < myId) ? 1u : 0u;
+  const uint32_t d = c0 + rank;
+  b.x1[d] = b.xp[s];
+  b.prev[d] = b.x0[s];  // demo4.cpp:334 prevPosition = curPosition
+  b.id2[d] = myId;
+  b.keys[d] = key;
+}
+
+// ---------------------------------------------------------------------------
+// 4. density + pressure
+// ---------------------------------------------------------------------------
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_density(Buffers b, DevParams p, DevScene s0, int n) {
+  const int i = (int)(xcd_block() * kBlock + threadIdx.x);
+  uint32_t cand = 0;
+  if (i < n) {
+    const uint32_t key = b.keys[i];
+    const DevScene sc = scene_of_slot<MULTI>(b, s0, key);
+    const uint32_t local = key - sc.cellBase;
+    const int cy = (int)(local / (uint32_t)sc.nx);
+    const int cx = (int)local - cy * sc.nx;
+    const float2 xi = b.x1[i];
+    float d0 = 0.0f, d1 = 0.0f;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const RowRange rr = row_range(b.startNext, sc, cx, cy, dy);
+      cand += rr.j1 - rr.j0;
+      for (uint32_t j = rr.j0; j < rr.j1; ++j) {
+        const float2 xj = b.x1[j];
+        const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;  // sph.h:466
+        const float r2 = Rx * Rx + Ry * Ry;               // sph.h:467
+        if (r2 < p.h2) {                                  // sph.h:470
+          const float r = sqrtf(r2);                      // sph.h:471
+          const float term = 1.0f - r * p.invH;           // sph.h:472
+          d0 = d0 + (term * term);                        // sph.h:473
+          d1 = d1 + ((term * term) * term);               // sph.h:474
+        }
+      }
+    }
+    b.dens[i] = make_float2(d0, d1);
+    b.pres[i] = make_float2(p.k * (d0 - p.rho0), p.kn * d1);  // sph.h:478-481
+  }
+  // candidate-count statistics (demo4.cpp:369-376)
+  uint32_t mn = (i < n) ? cand : 0xFFFFFFFFu, mx = cand;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
+    mn = min(mn, (uint32_t)__shfl_xor((int)mn, d, 64));
+  }
+  if ((threadIdx.x & 63) == 0 && mn != 0xFFFFFFFFu) {
+    atomicMin(&b.stats[0], mn);
+    atomicMax(&b.stats[1], mx);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 5. relaxation + collisions + velocity
+// ---------------------------------------------------------------------------
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_delta(Buffers b, DevParams p, DevScene s0, int n) {
+  const int i = (int)(xcd_block() * kBlock + threadIdx.x);
+  if (i >= n) return;
+  const uint32_t key = b.keys[i];
+  const DevScene sc = scene_of_slot<MULTI>(b, s0, key);
+  const uint32_t local = key - sc.cellBase;
+  const int cy = (int)(local / (uint32_t)sc.nx);
+  const int cx = (int)local - cy * sc.nx;
+  const float2 xi = b.x1[i];
+  const float2 pi = b.pres[i];
+  float accx = 0.0f, accy = 0.0f;
+  for (int dy = -1; dy <= 1; ++dy) {
+    const RowRange rr = row_range(b.startNext, sc, cx, cy, dy);
+    for (uint32_t j = rr.j0; j < rr.j1; ++j) {
+      const float2 xj = b.x1[j];
+      const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;    // sph.h:486
+      const float r2 = Rx * Rx + Ry * Ry;                 // sph.h:487
+      if (r2 < p.h2) {                                    // sph.h:488
+        const float2 pj = b.pres[j];
+        const float r = sqrtf(r2);                        // sph.h:489
+        const float l = (r == 0.0f) ? 1.0f : r;           // sph.h:490 Vec2Normalize
+        const float inv = 1.0f / l;
+        const float nX = Rx * inv, nY = Ry * inv;
+        const float term = 1.0f - r * p.invH;             // sph.h:491
+        const float tt = term * term;
+        const float dij = p.dt2 * (pi.x * term + pi.y * tt);  // sph.h:492 with P_i
+        const float dji = p.dt2 * (pj.x * term + pj.y * tt);  // sph.h:492 with P_j (j's visit)
+        // demo4.cpp:251 dx -= delta * 0.5f (own visit)
+        accx = accx - ((dij * nX) * 0.5f);
+        accy = accy - ((dij * nY) * 0.5f);
+        // demo4.cpp:250 neighbour += delta * 0.5f (j's visit; n_ji = -n_ij)
+        accx = accx - ((dji * nX) * 0.5f);
+        accy = accy - ((dji * nY) * 0.5f);
+      }
+    }
+  }
+  float x = accx + xi.x;                                  // demo4.cpp:253
+  float y = accy + xi.y;
+  collide(b, p, sc, x, y);                                // demo4.cpp:411-444
+  const float2 pv = b.prev[i];
+  b.x0[i] = make_float2(x, y);
+  b.v[i] = make_float2((x - pv.x) * p.invDt, (y - pv.y) * p.invDt);  // demo4.cpp:449
+}
+
+// ---------------------------------------------------------------------------
+// State readback in particle-index order
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_gather_state(Buffers b, DevScene s0, int n, int nBinned, float2 *x,
+                                                         float2 *prev, float2 *v, float2 *dens, float2 *pres,
+                                                         uint32_t *key, int32_t *scene) {
+  const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (i >= n) return;
+  const uint32_t id = b.id[i];
+  if (x) x[id] = b.x0[i];
+  if (prev) prev[id] = b.prev[i];
+  if (v) v[id] = b.v[i];
+  if (dens) dens[id] = b.dens[i];
+  if (pres) pres[id] = b.pres[i];
+  if (key) key[id] = b.keys[i];
+  if (scene) {
+    int s = 0;
+    if (b.nScenes > 1) s = (i < nBinned) ? scene_of_key(b.sceneBase, b.nScenes, b.keys[i]) : (int)b.scn[i];
+    scene[id] = s;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *p, uint32_t value, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 diagnostics: per-workgroup partial sums over fixed slot ranges, then one
+// workgroup combines them in workgroup order.
+// ---------------------------------------------------------------------------
+constexpr int kDiag = 12;  // vx, vy, |v|^2, rho, max rho, |rho-rho0|, (rho-rho0)^2, x, y, non-finite, count, pad
+constexpr int kDiagItems = 16;
+
+__device__ __forceinline__ double block_sum_d(double v, double *sh) {
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < kBlock / 64; ++w) t += sh[w];
+  return t;
+}
+__device__ __forceinline__ double block_max_d(double v, double *sh) {
+  for (int d = 32; d >= 1; d >>= 1) v = fmax(v, __shfl_xor(v, d, 64));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double t = sh[0];
+  for (int w = 1; w < kBlock / 64; ++w) t = fmax(t, sh[w]);
+  return t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_diag_partial(Buffers b, DevParams p, int n, double *partials) {
+  __shared__ double sh[kBlock / 64];
+  double acc[kDiag];
+#pragma unroll
+  for (int k = 0; k < kDiag; ++k) acc[k] = 0.0;
+  acc[4] = -1.0e300;
+  const double rho0 = (double)p.rho0;
+  const int base = blockIdx.x * kBlock * kDiagItems;
+  for (int k = 0; k < kDiagItems; ++k) {
+    const int i = base + k * kBlock + threadIdx.x;
+    if (i >= n) break;
+    const float2 v = b.v[i], x = b.x0[i], d = b.dens[i];
+    const bool finite = isfinite(v.x) && isfinite(v.y) && isfinite(x.x) && isfinite(x.y);
+    if (!finite) { acc[9] += 1.0; continue; }
+    const double vx = v.x, vy = v.y, rho = d.x;
+    acc[0] += vx;
+    acc[1] += vy;
+    acc[2] += vx * vx + vy * vy;
+    acc[3] += rho;
+    acc[4] = fmax(acc[4], rho);
+    acc[5] += fabs(rho - rho0);
+    acc[6] += (rho - rho0) * (rho - rho0);
+    acc[7] += (double)x.x;
+    acc[8] += (double)x.y;
+    acc[10] += 1.0;
+  }
+  double out[kDiag];
+#pragma unroll
+  for (int k = 0; k < kDiag; ++k) out[k] = (k == 4) ? block_max_d(acc[k], sh) : block_sum_d(acc[k], sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < kDiag; ++k) partials[(size_t)blockIdx.x * kDiag + k] = out[k];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_diag_final(int nblocks, const double *partials, double *out) {
+  if (threadIdx.x != 0) return;
+  double r[kDiag];
+  for (int k = 0; k < kDiag; ++k) r[k] = 0.0;
+  r[4] = -1.0e300;
+  for (int bI = 0; bI < nblocks; ++bI) {
+    for (int k = 0; k < kDiag; ++k) {
+      const double v = partials[(size_t)bI * kDiag + k];
+      r[k] = (k == 4) ? fmax(r[k], v) : r[k] + v;
+    }
+  }
+  for (int k = 0; k < kDiag; ++k) out[k] = r[k];
+}
+
+int diagnostics_blocks(int n) { return (n + kBlock * kDiagItems - 1) / (kBlock * kDiagItems); }
+
+// ---------------------------------------------------------------------------
+// Arithmetic self-test
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_selftest_math(const float *a, const float *b, uint64_t n, float *out) {
+  const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float x = a[i], y = b[i];
+  out[4 * i + 0] = sqrtf(x);
+  out[4 * i + 1] = 1.0f / x;
+  out[4 * i + 2] = x / y;
+  out[4 * i + 3] = (float)cell_coord(x, y, 1.0f, 1 << 20);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+void launch_unbin(const Buffers &b, int nBinned, hipStream_t s) {
+  if (nBinned <= 0) return;
+  hipLaunchKernelGGL(k_unbin, dim3(grid_for(nBinned)), dim3(kBlock), 0, s, b, nBinned);
+}
+
+void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int nBinned, int lists,
+                         hipStream_t s) {
+  if (n <= 0) return;
+  if (b.nScenes > 1)
+    hipLaunchKernelGGL(k_visc_predict<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n, nBinned, lists);
+  else
+    hipLaunchKernelGGL(k_visc_predict<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n, nBinned, lists);
+}
+
+void launch_scan(const Buffers &b, uint32_t cells, hipStream_t s) {
+  const uint32_t nb = (cells + kScanChunk - 1) / kScanChunk;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, b, cells);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kBlock), 0, s, b, nb);
+  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kBlock), 0, s, b, cells);
+}
+
+void launch_scatter_reorder(const Buffers &b, int n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, b, n);
+  hipLaunchKernelGGL(k_rank_reorder, dim3(grid_for(n)), dim3(kBlock), 0, s, b, n);
+}
+
+void launch_density(const Buffers &b, const DevParams &p, const DevScene &s0, int n, hipStream_t s) {
+  if (n <= 0) return;
+  if (b.nScenes > 1)
+    hipLaunchKernelGGL(k_density<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n);
+  else
+    hipLaunchKernelGGL(k_density<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n);
+}
+
+void launch_delta(const Buffers &b, const DevParams &p, const DevScene &s0, int n, hipStream_t s) {
+  if (n <= 0) return;
+  if (b.nScenes > 1)
+    hipLaunchKernelGGL(k_delta<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n);
+  else
+    hipLaunchKernelGGL(k_delta<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n);
+}
+
+void launch_gather_state(const Buffers &b, const DevScene &s0, int n, int nBinned, float2 *x, float2 *prev, float2 *v,
+                         float2 *dens, float2 *pres, uint32_t *key, int32_t *scene, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather_state, dim3(grid_for(n)), dim3(kBlock), 0, s, b, s0, n, nBinned, x, prev, v, dens, pres,
+                     key, scene);
+}
+
+void launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n)), dim3(kBlock), 0, s, p, value, n);
+}
+
+void launch_diagnostics(const Buffers &b, const DevParams &p, int n, double *partials, double *out, hipStream_t s) {
+  const int nb = diagnostics_blocks(n);
+  if (nb > 0) hipLaunchKernelGGL(k_diag_partial, dim3(nb), dim3(kBlock), 0, s, b, p, n, partials);
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kBlock), 0, s, nb, partials, out);
+}
+
+void launch_selftest_math(const float *a, const float *b, uint64_t n, float *out, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_selftest_math, dim3(grid_for(n)), dim3(kBlock), 0, s, a, b, n, out);
+}
+
+}  // namespace nbs

@@ -1,0 +1,90 @@
+// Shared host/device declarations of the MI355X SPH step kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nbs {
+
+constexpr int kBlock = 256;          // threads per workgroup of the particle kernels
+constexpr int kScanItems = 8;        // cells per thread in the scan kernels
+constexpr int kScanChunk = kBlock * kScanItems;
+
+// Step constants, derived on the host in FP32 exactly as the reference derives
+// them (sph.h:92 invKernelHeight, sph.h:470 h*h, sph.h:492 dt*dt,
+// demo4.cpp:287 invDt, demo4.cpp:306 gravity + externalForce).
+struct DevParams {
+  float h, invH, h2, rho0, k, kn, sigma, beta, R, margin;
+  float dt, dt2, invDt;
+  float gex, gey;    // gravity + external force
+  float gdtx, gdty;  // ((gravity + external) + 0) * dt: velocity increment of a particle without own force
+};
+
+// One scene: an origin-centred box and its grid.  Keys of the scene's cells
+// are cellBase + cy * nx + cx (sph.h:444-448).
+struct DevScene {
+  float halfW, halfH, cellSize;
+  int nx, ny;
+  uint32_t cellBase;
+  int bodyBegin, bodyEnd;
+};
+
+enum BodyType : int { BODY_PLANE = 1, BODY_CIRCLE = 2, BODY_SEGMENT = 3, BODY_POLYGON = 4 };
+
+// plane: (a0, a1) normal, a2 distance; circle: (a0, a1) centre, a2 radius;
+// segment: (a0, a1)-(a2, a3); polygon: nverts vertices at verts[vertOffset].
+struct DevBody {
+  int type;
+  int nverts;
+  int vertOffset;
+  int pad;
+  float a0, a1, a2, a3;
+};
+
+// Device storage.  Particle arrays are indexed by slot: slots [0, nBinned)
+// hold the particles of the last binning sorted by (cell key, particle index);
+// slots [nBinned, n) hold particles added since, in index order.
+struct Buffers {
+  float2 *x0;    // positions (after the last step)
+  float2 *xp;    // predicted positions, slot order of the previous binning
+  float2 *x1;    // predicted positions, slot order of the new binning
+  float2 *prev;  // positions before predict (new binning order)
+  float2 *v;     // velocities
+  float2 *dens;  // (density, nearDensity)
+  float2 *pres;  // (pressure, nearPressure)
+  float2 *acc;   // own acceleration of particles not yet stepped
+  uint32_t *id, *id2;  // particle index per slot (current / next)
+  uint32_t *keys;      // cell key of the last binning per slot
+  uint32_t *keyp;      // cell key of the predicted position (previous slot order)
+  uint32_t *fill;      // arrival index within the new cell
+  uint32_t *tmpSlot, *tmpId;  // cell-bucketed slots and particle indices
+  uint32_t *scn;       // scene of particles not yet binned
+  uint32_t *count;     // per cell, zero between steps
+  uint32_t *startCur;  // per cell + 1: the last binning
+  uint32_t *startNext; // per cell + 1: the binning being built
+  uint32_t *blockSum, *blockOff;
+  uint32_t *stats;     // min/max candidates, min/max occupied-cell count
+  const DevScene *scenes;
+  const uint32_t *sceneBase;  // cellBase of each scene, ascending
+  int nScenes;
+  const DevBody *bodies;
+  const float2 *verts;
+};
+
+// Launchers (nbs_kernels.hip).  All kernels are enqueued on `stream`.
+void launch_unbin(const Buffers &b, int nBinned, hipStream_t stream);
+void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int nBinned, int lists,
+                         hipStream_t stream);
+void launch_scan(const Buffers &b, uint32_t cells, hipStream_t stream);
+void launch_scatter_reorder(const Buffers &b, int n, hipStream_t stream);
+void launch_density(const Buffers &b, const DevParams &p, const DevScene &s0, int n, hipStream_t stream);
+void launch_delta(const Buffers &b, const DevParams &p, const DevScene &s0, int n, hipStream_t stream);
+void launch_gather_state(const Buffers &b, const DevScene &s0, int n, int nBinned, float2 *x, float2 *prev, float2 *v,
+                         float2 *dens, float2 *pres, uint32_t *key, int32_t *scene, hipStream_t stream);
+void launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t stream);
+void launch_diagnostics(const Buffers &b, const DevParams &p, int n, double *partials, double *out,
+                        hipStream_t stream);
+int diagnostics_blocks(int n);
+void launch_selftest_math(const float *a, const float *b, uint64_t n, float *out, hipStream_t stream);
+
+}  // namespace nbs

@@ -1,0 +1,1016 @@
+// Host side of the MI355X SPH step library: device storage, step scheduling,
+// HIP-graph replay, timing, and the C ABI declared in include/nbs_sph.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/nbs_sph.h"
+#include "nbs_kernels.hpp"
+
+namespace nbs {
+
+static thread_local std::string g_lastError;
+
+struct Error : std::runtime_error {
+  nbs_status code;
+  Error(nbs_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define NBS_HIP(expr)                                                                                  \
+  do {                                                                                                 \
+    hipError_t e_ = (expr);                                                                            \
+    if (e_ != hipSuccess) throw Error(NBS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error(NBS_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    n = count;
+  }
+  void swap(DevBuf &o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+  }
+};
+
+struct HostBody {
+  int type;
+  float a[4];
+  std::vector<float> verts;
+};
+
+struct HostScene {
+  nbs_domain dom;
+  float halfW, halfH;
+  int nx, ny;
+  uint32_t cellBase;
+  std::vector<HostBody> bodies;
+};
+
+struct Emitter {  // demo4.h:123-135
+  int scene;
+  float px, py, dx, dy, radius, speed, rate, duration, elapsed, totalElapsed;
+  bool active;
+  uint64_t seed, draws;
+};
+
+// SplitMix64 uniform in [-1, 1) with 24-bit resolution (exact in FP32).
+static inline float splitmix_uniform(uint64_t seed, uint64_t counter) {
+  uint64_t z = seed + (counter + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  const float k = (float)(uint32_t)(z >> 40);
+  return k * (1.0f / 8388608.0f) - 1.0f;
+}
+
+class Simulation {
+ public:
+  Simulation(const nbs_params &p, const nbs_domain &d, uint64_t capacity) : params_(p) {
+    validateParams(p);
+    int dev = 0;
+    NBS_HIP(hipGetDevice(&dev));
+    device_ = dev;
+    NBS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    NBS_HIP(hipMalloc((void **)&stats_, 4 * sizeof(uint32_t)));
+    resetStats();
+    addScene(d);
+    reserve(std::max<uint64_t>(capacity, 1024));
+  }
+  ~Simulation() {
+    destroyGraph();
+    for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+    if (stats_) (void)hipFree(stats_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  // ---- parameters -------------------------------------------------------
+  static void validateParams(const nbs_params &p) {
+    if (!(p.kernel_height > 0.0f) || !std::isfinite(p.kernel_height))
+      throw Error(NBS_ERR_INVALID_ARGUMENT, "kernel_height must be positive and finite");
+    const float v[] = {p.rest_density, p.stiffness, p.near_stiffness, p.linear_viscosity, p.quadratic_viscosity,
+                       p.collision_radius, p.collision_margin, p.particle_spacing, p.cell_size};
+    for (float x : v)
+      if (!std::isfinite(x)) throw Error(NBS_ERR_INVALID_ARGUMENT, "parameters must be finite");
+  }
+  void setParams(const nbs_params &p) {
+    validateParams(p);
+    params_ = p;
+  }
+  const nbs_params &params() const { return params_; }
+  void setGravity(float gx, float gy) { gravity_[0] = gx, gravity_[1] = gy; }
+  void addExternalForce(float fx, float fy) {  // demo4.h:189-191 externalForce += force
+    ext_[0] = fx + ext_[0];
+    ext_[1] = fy + ext_[1];
+  }
+  void clearExternalForce() { ext_[0] = ext_[1] = 0.0f; }
+
+  // ---- scenes and bodies -------------------------------------------------
+  int addScene(const nbs_domain &d) {
+    if (!(d.width > 0.0f) || !(d.height > 0.0f) || !(d.cell_size > 0.0f) || !std::isfinite(d.width) ||
+        !std::isfinite(d.height) || !std::isfinite(d.cell_size))
+      throw Error(NBS_ERR_INVALID_ARGUMENT, "domain sizes must be positive and finite");
+    HostScene s;
+    s.dom = d;
+    s.halfW = d.width * 0.5f;  // sph.h:21-22
+    s.halfH = d.height * 0.5f;
+    const double fx = std::floor((double)(d.width / d.cell_size)), fy = std::floor((double)(d.height / d.cell_size));
+    if (fx > 65536.0 || fy > 65536.0) throw Error(NBS_ERR_CAPACITY, "grid larger than 65536 cells per axis");
+    s.nx = std::max(1, (int)(d.width / d.cell_size));  // sph.h:61-62
+    s.ny = std::max(1, (int)(d.height / d.cell_size));
+    const uint64_t total = (uint64_t)totalCells_ + (uint64_t)s.nx * (uint64_t)s.ny;
+    if (total > (1ull << 31)) throw Error(NBS_ERR_CAPACITY, "more than 2^31 cells in total");
+    s.cellBase = totalCells_;
+    scenes_.push_back(s);
+    totalCells_ = (uint32_t)total;
+    allocCells();
+    bodiesDirty_ = true;
+    scenesDirty_ = true;
+    // The cell tables change shape: the next step starts without lists.
+    unbinAll();
+    return (int)scenes_.size() - 1;
+  }
+  int sceneCount() const { return (int)scenes_.size(); }
+  void checkScene(int s) const {
+    if (s < 0 || s >= (int)scenes_.size()) throw Error(NBS_ERR_INVALID_ARGUMENT, "scene index out of range");
+  }
+  void clearBodies() {
+    for (auto &s : scenes_) s.bodies.clear();
+    bodiesDirty_ = true;
+  }
+  void addBody(int scene, HostBody b) {
+    checkScene(scene);
+    scenes_[scene].bodies.push_back(std::move(b));
+    bodiesDirty_ = true;
+  }
+
+  // ---- particles ----------------------------------------------------------
+  uint64_t count() const { return n_; }
+  uint64_t addParticles(int scene, const float *pos, const float *force, uint64_t k) {
+    checkScene(scene);
+    if (k == 0) return n_;
+    if (!pos) throw Error(NBS_ERR_INVALID_ARGUMENT, "positions are NULL");
+    if (n_ + k > (uint64_t)INT32_MAX) throw Error(NBS_ERR_CAPACITY, "more than 2^31-1 particles");
+    reserve(n_ + k);
+    const HostScene &sc = scenes_[scene];
+    std::vector<float> zero(2 * k, 0.0f), acc(2 * k, 0.0f);
+    std::vector<uint32_t> ids(k), keys(k), scn(k, (uint32_t)scene);
+    if (force) std::memcpy(acc.data(), force, 2 * k * sizeof(float));
+    for (uint64_t q = 0; q < k; ++q) {
+      ids[q] = (uint32_t)(n_ + q);
+      keys[q] = hostKey(sc, pos[2 * q], pos[2 * q + 1]);  // demo4.cpp:151 InsertParticleIntoGrid
+    }
+    const size_t o = n_;
+    copyIn(buf_.x0 + o, pos, 2 * k);
+    copyIn(buf_.prev + o, pos, 2 * k);  // demo4.h:101-106 ParticleData(pos)
+    copyIn(buf_.v + o, zero.data(), 2 * k);
+    copyIn(buf_.dens + o, zero.data(), 2 * k);
+    copyIn(buf_.pres + o, zero.data(), 2 * k);
+    copyIn(buf_.acc + o, acc.data(), 2 * k);
+    NBS_HIP(hipMemcpyAsync(buf_.id + o, ids.data(), k * 4, hipMemcpyHostToDevice, stream_));
+    NBS_HIP(hipMemcpyAsync(buf_.keys + o, keys.data(), k * 4, hipMemcpyHostToDevice, stream_));
+    NBS_HIP(hipMemcpyAsync(buf_.scn + o, scn.data(), k * 4, hipMemcpyHostToDevice, stream_));
+    NBS_HIP(hipStreamSynchronize(stream_));
+    const uint64_t first = n_;
+    n_ += k;
+    destroyGraph();
+    return first;
+  }
+  void clearParticles() {
+    n_ = 0;
+    nBinned_ = 0;
+    lists_ = false;
+    destroyGraph();
+  }
+  // demo4.cpp:169-181 AddVolume with seeded per-axis jitter.
+  void addVolume(int scene, float cx, float cy, float fx, float fy, int countX, int countY, float spacing,
+                 uint64_t seed) {
+    checkScene(scene);
+    if (countX < 0 || countY < 0) throw Error(NBS_ERR_INVALID_ARGUMENT, "negative volume count");
+    const uint64_t k = (uint64_t)countX * (uint64_t)countY;
+    if (k == 0) return;
+    std::vector<float> pos(2 * k), force(2 * k);
+    const float offX = ((float)countX * spacing) * 0.5f;  // demo4.cpp:170
+    const float offY = ((float)countY * spacing) * 0.5f;
+    const float h = params_.kernel_height;
+    uint64_t q = 0;
+    for (int iy = 0; iy < countY; ++iy) {
+      for (int ix = 0; ix < countX; ++ix, ++q) {
+        float px = (float)ix * spacing, py = (float)iy * spacing;  // demo4.cpp:173
+        const float half = spacing * 0.5f;
+        px = half + px;                                           // demo4.cpp:174
+        py = half + py;
+        px = (cx - offX) + px;                                    // demo4.cpp:175
+        py = (cy - offY) + py;
+        const float jx = (splitmix_uniform(seed, 2 * q) * h) * 0.01f;  // demo4.cpp:176
+        const float jy = (splitmix_uniform(seed, 2 * q + 1) * h) * 0.01f;
+        pos[2 * q] = jx + px;                                     // demo4.cpp:177
+        pos[2 * q + 1] = jy + py;
+        force[2 * q] = fx;
+        force[2 * q + 1] = fy;
+      }
+    }
+    addParticles(scene, pos.data(), force.data(), k);
+  }
+  void addEmitter(const Emitter &e) {
+    checkScene(e.scene);
+    emitters_.push_back(e);
+  }
+  void clearEmitters() { emitters_.clear(); }
+
+  // ---- state ------------------------------------------------------------
+  void getState(float *x, float *prev, float *v, float *dens, float *pres, uint32_t *key, int32_t *scene) {
+    if (n_ == 0) return;
+    ensureStaging();
+    launch_gather_state(buf_, devScene0(), (int)n_, (int)nBinned_, x ? stX_.p : nullptr, prev ? stPrev_.p : nullptr,
+                        v ? stV_.p : nullptr, dens ? stDens_.p : nullptr, pres ? stPres_.p : nullptr,
+                        key ? stKey_.p : nullptr, scene ? stScene_.p : nullptr, stream_);
+    NBS_HIP(hipGetLastError());
+    if (x) copyOut(x, stX_.p, 2 * n_);
+    if (prev) copyOut(prev, stPrev_.p, 2 * n_);
+    if (v) copyOut(v, stV_.p, 2 * n_);
+    if (dens) copyOut(dens, stDens_.p, 2 * n_);
+    if (pres) copyOut(pres, stPres_.p, 2 * n_);
+    if (key) NBS_HIP(hipMemcpyAsync(key, stKey_.p, n_ * 4, hipMemcpyDeviceToHost, stream_));
+    if (scene) NBS_HIP(hipMemcpyAsync(scene, stScene_.p, n_ * 4, hipMemcpyDeviceToHost, stream_));
+    NBS_HIP(hipStreamSynchronize(stream_));
+  }
+  void setState(const float *x, const float *v) {
+    if (n_ == 0) return;
+    if (!x || !v) throw Error(NBS_ERR_INVALID_ARGUMENT, "x and v are required");
+    std::vector<int32_t> scene(n_);
+    getState(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, scene.data());
+    std::vector<uint32_t> ids(n_), keys(n_), scn(n_);
+    std::vector<float> zero(2 * n_, 0.0f);
+    for (uint64_t i = 0; i < n_; ++i) {
+      ids[i] = (uint32_t)i;
+      scn[i] = (uint32_t)scene[i];
+      keys[i] = hostKey(scenes_[scene[i]], x[2 * i], x[2 * i + 1]);
+    }
+    copyIn(buf_.x0, x, 2 * n_);
+    copyIn(buf_.prev, x, 2 * n_);
+    copyIn(buf_.v, v, 2 * n_);
+    copyIn(buf_.acc, zero.data(), 2 * n_);
+    copyIn(buf_.dens, zero.data(), 2 * n_);
+    copyIn(buf_.pres, zero.data(), 2 * n_);
+    NBS_HIP(hipMemcpyAsync(buf_.id, ids.data(), n_ * 4, hipMemcpyHostToDevice, stream_));
+    NBS_HIP(hipMemcpyAsync(buf_.keys, keys.data(), n_ * 4, hipMemcpyHostToDevice, stream_));
+    NBS_HIP(hipMemcpyAsync(buf_.scn, scn.data(), n_ * 4, hipMemcpyHostToDevice, stream_));
+    NBS_HIP(hipStreamSynchronize(stream_));
+    nBinned_ = 0;
+    lists_ = false;
+    destroyGraph();
+  }
+  void getGrid(uint32_t *sortedIds, uint32_t *cellStart) {
+    if (sortedIds && n_) NBS_HIP(hipMemcpyAsync(sortedIds, buf_.id, n_ * 4, hipMemcpyDeviceToHost, stream_));
+    if (cellStart)
+      NBS_HIP(hipMemcpyAsync(cellStart, buf_.startCur, ((size_t)totalCells_ + 1) * 4, hipMemcpyDeviceToHost, stream_));
+    NBS_HIP(hipStreamSynchronize(stream_));
+  }
+  uint64_t totalCells() const { return totalCells_; }
+
+  // ---- statistics ---------------------------------------------------------
+  void resetStats() {
+    std::memset(&stats_h_, 0, sizeof(stats_h_));
+    stats_h_.min_particle_neighbor_count = 500;  // sph.h:143-147
+    stats_h_.min_cell_particle_count = 500;
+    for (int k = 0; k < NBS_KERNEL_COUNT; ++k) kernelMs_[k] = 0.0, kernelLaunches_[k] = 0;
+  }
+  nbs_stats stats() {
+    if (statsPending_) {
+      uint32_t s[4];
+      NBS_HIP(hipMemcpy(s, stats_, sizeof(s), hipMemcpyDeviceToHost));
+      stats_h_.min_particle_neighbor_count = s[0];
+      stats_h_.max_particle_neighbor_count = s[1];
+      stats_h_.min_cell_particle_count = (s[2] == 0xFFFFFFFFu) ? 0 : s[2];
+      stats_h_.max_cell_particle_count = s[3];
+      statsPending_ = false;
+    }
+    return stats_h_;
+  }
+  void kernelTimes(double *avg, uint64_t *launches) const {
+    for (int k = 0; k < NBS_KERNEL_COUNT; ++k) {
+      if (avg) avg[k] = kernelLaunches_[k] ? kernelMs_[k] / (double)kernelLaunches_[k] : 0.0;
+      if (launches) launches[k] = kernelLaunches_[k];
+    }
+  }
+  void setTiming(bool on) {
+    timing_ = on;
+    destroyGraph();
+  }
+  void setOptions(uint32_t f) {
+    options_ = f;
+    destroyGraph();
+  }
+
+  nbs_diagnostics diagnostics() {
+    nbs_diagnostics d{};
+    d.particle_count = n_;
+    if (n_ == 0) return d;
+    const int nb = diagnostics_blocks((int)n_);
+    if (diagPartials_.n < (size_t)nb * 12) diagPartials_.alloc((size_t)nb * 12);
+    if (diagOut_.n < 12) diagOut_.alloc(12);
+    launch_diagnostics(buf_, devParams(0.0f), (int)n_, diagPartials_.p, diagOut_.p, stream_);
+    NBS_HIP(hipGetLastError());
+    double r[12];
+    NBS_HIP(hipMemcpyAsync(r, diagOut_.p, sizeof(r), hipMemcpyDeviceToHost, stream_));
+    NBS_HIP(hipStreamSynchronize(stream_));
+    const double cnt = r[10] > 0 ? r[10] : 1.0;
+    const double rho0 = params_.rest_density;
+    d.momentum[0] = r[0];
+    d.momentum[1] = r[1];
+    d.kinetic_energy = 0.5 * r[2];
+    d.mean_density = r[3] / cnt;
+    d.max_density = r[4];
+    d.mean_abs_density_error = rho0 != 0.0 ? (r[5] / cnt) / std::fabs(rho0) : 0.0;
+    d.rms_density_error = rho0 != 0.0 ? std::sqrt(r[6] / cnt) / std::fabs(rho0) : 0.0;
+    d.centroid[0] = r[7] / cnt;
+    d.centroid[1] = r[8] / cnt;
+    d.non_finite_count = (uint64_t)r[9];
+    return d;
+  }
+
+  // ---- the step -----------------------------------------------------------
+  void update(float dt, uint32_t nsteps) {
+    if (!(dt > 0.0f) || !std::isfinite(dt)) throw Error(NBS_ERR_INVALID_ARGUMENT, "dt must be positive and finite");
+    syncBodies();
+    if (timing_) ensureEvents(nsteps);
+    uint32_t s = 0;
+    while (s < nsteps) {
+      if (!emitters_.empty()) runEmitters(dt);  // demo4.cpp:290-299
+      if (n_ == 0) {
+        ++s;
+        continue;
+      }
+      const bool graphable = (options_ & NBS_OPT_HIP_GRAPH) && !timing_ && emitters_.empty() && lists_ &&
+                             nBinned_ == n_;
+      if (graphable) {
+        // Replay pairs of steps (the cell tables and the slot-id arrays
+        // alternate between two buffers every step).
+        const uint32_t pairs = (nsteps - s) / 2;
+        if (pairs > 0) {
+          ensureGraph(dt);
+          for (uint32_t q = 0; q < pairs; ++q) NBS_HIP(hipGraphLaunch(graphExec_, stream_));
+          s += 2 * pairs;
+          statsPending_ = true;
+          continue;
+        }
+      }
+      enqueueStep(dt, timing_ ? (int)s : -1);
+      swapAfterStep();
+      ++s;
+    }
+    NBS_HIP(hipGetLastError());
+    NBS_HIP(hipStreamSynchronize(stream_));
+    if (timing_ && nsteps > 0) collectTimes(nsteps);
+    statsPending_ = true;
+    if (options_ & NBS_OPT_CHECK_FINITE) {
+      nbs_diagnostics d = diagnostics();
+      if (d.non_finite_count) throw Error(NBS_ERR_NON_FINITE, "particle state holds NaN or Inf");
+    }
+  }
+
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  // FP32 derivations in the reference's order.
+  DevParams devParams(float dt) const {
+    DevParams p;
+    p.h = params_.kernel_height;
+    p.invH = 1.0f / p.h;                       // sph.h:92
+    p.h2 = p.h * p.h;                          // sph.h:470
+    p.rho0 = params_.rest_density;
+    p.k = params_.stiffness;
+    p.kn = params_.near_stiffness;
+    p.sigma = params_.linear_viscosity;
+    p.beta = params_.quadratic_viscosity;
+    p.R = params_.collision_radius;
+    p.margin = params_.collision_margin;
+    p.dt = dt;
+    p.dt2 = dt * dt;                           // sph.h:492
+    p.invDt = (dt != 0.0f) ? 1.0f / dt : 0.0f; // demo4.cpp:287
+    p.gex = gravity_[0] + ext_[0];             // demo4.cpp:306 gravity + externalForce
+    p.gey = gravity_[1] + ext_[1];
+    p.gdtx = (p.gex + 0.0f) * dt;              // demo4.cpp:306-307 with a zero own acceleration
+    p.gdty = (p.gey + 0.0f) * dt;
+    return p;
+  }
+  DevScene devScene(const HostScene &s, int bodyBegin, int bodyEnd) const {
+    DevScene d;
+    d.halfW = s.halfW;
+    d.halfH = s.halfH;
+    d.cellSize = s.dom.cell_size;
+    d.nx = s.nx;
+    d.ny = s.ny;
+    d.cellBase = s.cellBase;
+    d.bodyBegin = bodyBegin;
+    d.bodyEnd = bodyEnd;
+    return d;
+  }
+  DevScene devScene0() const { return hostScenes_.empty() ? devScene(scenes_[0], 0, 0) : hostScenes_[0]; }
+
+  static int hostCoord(float p, float half, float cs, int n) {  // sph.h:450-463
+    const float f = (p + half) / cs;
+    if (!(f >= 0.0f)) return 0;
+    if (f >= (float)n) return n - 1;
+    return (int)f;
+  }
+  static uint32_t hostKey(const HostScene &s, float x, float y) {
+    const int cx = hostCoord(x, s.halfW, s.dom.cell_size, s.nx);
+    const int cy = hostCoord(y, s.halfH, s.dom.cell_size, s.ny);
+    return s.cellBase + (uint32_t)(cy * s.nx + cx);
+  }
+
+  void copyIn(float2 *dst, const float *src, size_t floats) {
+    NBS_HIP(hipMemcpyAsync(dst, src, floats * sizeof(float), hipMemcpyHostToDevice, stream_));
+  }
+  void copyOut(float *dst, const float2 *src, size_t floats) {
+    NBS_HIP(hipMemcpyAsync(dst, src, floats * sizeof(float), hipMemcpyDeviceToHost, stream_));
+  }
+
+  void reserve(uint64_t want) {
+    if (want <= cap_) return;
+    const uint64_t cap = std::max<uint64_t>(want, cap_ * 2);
+    // persistent arrays keep their contents
+    auto grow2 = [&](DevBuf<float2> &b) {
+      DevBuf<float2> nb;
+      nb.alloc(cap);
+      if (b.p && n_) NBS_HIP(hipMemcpyAsync(nb.p, b.p, n_ * sizeof(float2), hipMemcpyDeviceToDevice, stream_));
+      NBS_HIP(hipStreamSynchronize(stream_));
+      b.swap(nb);
+    };
+    auto grow4 = [&](DevBuf<uint32_t> &b) {
+      DevBuf<uint32_t> nb;
+      nb.alloc(cap);
+      if (b.p && n_) NBS_HIP(hipMemcpyAsync(nb.p, b.p, n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_));
+      NBS_HIP(hipStreamSynchronize(stream_));
+      b.swap(nb);
+    };
+    grow2(d_.x0); grow2(d_.prev); grow2(d_.v); grow2(d_.dens); grow2(d_.pres); grow2(d_.acc);
+    grow4(d_.id); grow4(d_.keys); grow4(d_.scn);
+    d_.xp.alloc(cap); d_.x1.alloc(cap); d_.id2.alloc(cap); d_.keyp.alloc(cap); d_.fill.alloc(cap);
+    d_.tmpSlot.alloc(cap); d_.tmpId.alloc(cap);
+    cap_ = cap;
+    rebindBuffers();
+    destroyGraph();
+  }
+  void allocCells() {
+    const size_t c = totalCells_;
+    const size_t nb = (c + kScanChunk - 1) / kScanChunk;
+    d_.count.alloc(c);
+    d_.startCur.alloc(c + 1);
+    d_.startNext.alloc(c + 1);
+    d_.blockSum.alloc(std::max<size_t>(nb, 1));
+    d_.blockOff.alloc(std::max<size_t>(nb, 1));
+    launch_fill_u32(d_.count.p, 0u, c, stream_);
+    launch_fill_u32(d_.startCur.p, 0u, c + 1, stream_);
+    launch_fill_u32(d_.startNext.p, 0u, c + 1, stream_);
+    NBS_HIP(hipGetLastError());
+    NBS_HIP(hipStreamSynchronize(stream_));
+    rebindBuffers();
+    destroyGraph();
+  }
+  void unbinAll() {
+    if (nBinned_ > 0) {
+      syncBodies();
+      launch_unbin(buf_, (int)nBinned_, stream_);
+      NBS_HIP(hipGetLastError());
+      NBS_HIP(hipStreamSynchronize(stream_));
+    }
+    nBinned_ = 0;
+    lists_ = false;
+    destroyGraph();
+  }
+  void ensureStaging() {
+    if (stX_.n >= n_) return;
+    const size_t c = cap_;
+    stX_.alloc(c); stPrev_.alloc(c); stV_.alloc(c); stDens_.alloc(c); stPres_.alloc(c);
+    stKey_.alloc(c); stScene_.alloc(c);
+  }
+  void rebindBuffers() {
+    buf_.x0 = d_.x0.p; buf_.xp = d_.xp.p; buf_.x1 = d_.x1.p; buf_.prev = d_.prev.p; buf_.v = d_.v.p;
+    buf_.dens = d_.dens.p; buf_.pres = d_.pres.p; buf_.acc = d_.acc.p;
+    buf_.id = d_.id.p; buf_.id2 = d_.id2.p; buf_.keys = d_.keys.p; buf_.keyp = d_.keyp.p; buf_.fill = d_.fill.p;
+    buf_.tmpSlot = d_.tmpSlot.p; buf_.tmpId = d_.tmpId.p; buf_.scn = d_.scn.p;
+    buf_.count = d_.count.p; buf_.startCur = d_.startCur.p; buf_.startNext = d_.startNext.p;
+    buf_.blockSum = d_.blockSum.p; buf_.blockOff = d_.blockOff.p; buf_.stats = stats_;
+    buf_.scenes = devScenes_.p; buf_.sceneBase = devSceneBase_.p; buf_.nScenes = (int)scenes_.size();
+    buf_.bodies = devBodies_.p; buf_.verts = devVerts_.p;
+  }
+  // Uploads scene descriptors and the body table when they changed.
+  void syncBodies() {
+    if (!bodiesDirty_ && !scenesDirty_) return;
+    std::vector<DevBody> bodies;
+    std::vector<float2> verts;
+    hostScenes_.clear();
+    std::vector<uint32_t> bases;
+    for (const HostScene &s : scenes_) {
+      const int b0 = (int)bodies.size();
+      for (const HostBody &hb : s.bodies) {
+        DevBody d{};
+        d.type = hb.type;
+        d.a0 = hb.a[0]; d.a1 = hb.a[1]; d.a2 = hb.a[2]; d.a3 = hb.a[3];
+        if (hb.type == BODY_POLYGON) {
+          d.nverts = (int)(hb.verts.size() / 2);
+          d.vertOffset = (int)verts.size();
+          for (size_t k = 0; k < hb.verts.size(); k += 2) verts.push_back(make_float2(hb.verts[k], hb.verts[k + 1]));
+        }
+        bodies.push_back(d);
+      }
+      hostScenes_.push_back(devScene(s, b0, (int)bodies.size()));
+      bases.push_back(s.cellBase);
+    }
+    if (devBodies_.n < std::max<size_t>(bodies.size(), 1)) devBodies_.alloc(std::max<size_t>(bodies.size(), 16));
+    if (devVerts_.n < std::max<size_t>(verts.size(), 1)) devVerts_.alloc(std::max<size_t>(verts.size(), 16));
+    if (devScenes_.n < hostScenes_.size()) devScenes_.alloc(hostScenes_.size() * 2);
+    if (devSceneBase_.n < bases.size()) devSceneBase_.alloc(bases.size() * 2);
+    if (!bodies.empty())
+      NBS_HIP(hipMemcpyAsync(devBodies_.p, bodies.data(), bodies.size() * sizeof(DevBody), hipMemcpyHostToDevice, stream_));
+    if (!verts.empty())
+      NBS_HIP(hipMemcpyAsync(devVerts_.p, verts.data(), verts.size() * sizeof(float2), hipMemcpyHostToDevice, stream_));
+    NBS_HIP(hipMemcpyAsync(devScenes_.p, hostScenes_.data(), hostScenes_.size() * sizeof(DevScene),
+                           hipMemcpyHostToDevice, stream_));
+    NBS_HIP(hipMemcpyAsync(devSceneBase_.p, bases.data(), bases.size() * 4, hipMemcpyHostToDevice, stream_));
+    NBS_HIP(hipStreamSynchronize(stream_));
+    bodiesDirty_ = scenesDirty_ = false;
+    rebindBuffers();
+    destroyGraph();
+  }
+
+  // One step on stream_ (ev >= 0: record timing events for step slot ev).
+  void enqueueStep(float dt, int ev) {
+    const DevParams p = devParams(dt);
+    const DevScene s0 = hostScenes_[0];
+    const int n = (int)n_;
+    if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 0], stream_));
+    launch_visc_predict(buf_, p, s0, n, (int)nBinned_, lists_ ? 1 : 0, stream_);
+    if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 1], stream_));
+    launch_scan(buf_, totalCells_, stream_);
+    launch_scatter_reorder(buf_, n, stream_);
+    if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 2], stream_));
+    launch_density(buf_, p, s0, n, stream_);
+    if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 3], stream_));
+    launch_delta(buf_, p, s0, n, stream_);
+    if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 4], stream_));
+  }
+  void swapAfterStep() {
+    std::swap(buf_.id, buf_.id2);
+    d_.id.swap(d_.id2);
+    std::swap(buf_.startCur, buf_.startNext);
+    d_.startCur.swap(d_.startNext);
+    nBinned_ = n_;
+    lists_ = true;
+  }
+
+  void ensureEvents(uint32_t nsteps) {
+    const size_t want = 5 * (size_t)std::max<uint32_t>(nsteps, 1);
+    while (events_.size() < want) {
+      hipEvent_t e;
+      NBS_HIP(hipEventCreate(&e));
+      events_.push_back(e);
+    }
+  }
+  void collectTimes(uint32_t nsteps) {
+    float last[4] = {0, 0, 0, 0};
+    for (uint32_t s = 0; s < nsteps; ++s) {
+      for (int k = 0; k < 4; ++k) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, events_[5 * s + k], events_[5 * s + k + 1]) != hipSuccess) continue;
+        kernelMs_[k] += ms;
+        kernelLaunches_[k] += 1;
+        last[k] = ms;
+      }
+    }
+    stats_h_.time_emitters = 0.0f;
+    stats_h_.time_integration = 0.0f;
+    stats_h_.time_viscosity_forces = last[0];
+    stats_h_.time_predict = 0.0f;
+    stats_h_.time_update_grid = last[1];
+    stats_h_.time_neighbor_search = 0.0f;
+    stats_h_.time_density_and_pressure = last[2];
+    stats_h_.time_delta_positions = last[3];
+    stats_h_.time_collisions = 0.0f;
+  }
+
+  // Captures two steps (both buffer parities) as one graph.
+  void ensureGraph(float dt) {
+    const DevParams p = devParams(dt);
+    if (graphExec_ && std::memcmp(&p, &graphParams_, sizeof(p)) == 0 && graphN_ == n_) return;
+    destroyGraph();
+    const Buffers saved = buf_;
+    NBS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    enqueueStep(dt, -1);
+    swapAfterStep();
+    enqueueStep(dt, -1);
+    swapAfterStep();
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(stream_, &g);
+    if (e != hipSuccess) throw Error(NBS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&graphExec_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      graphExec_ = nullptr;
+      throw Error(NBS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    // Two swaps restore the buffer roles.
+    if (buf_.id != saved.id || buf_.startCur != saved.startCur) throw Error(NBS_ERR_HIP, "graph capture parity");
+    graphParams_ = p;
+    graphN_ = n_;
+  }
+  void destroyGraph() {
+    if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
+    graphExec_ = nullptr;
+  }
+
+  // demo4.cpp:257-284 UpdateEmitter
+  void runEmitters(float dt) {
+    const float spacing = params_.particle_spacing;
+    const float invDt = 1.0f / dt;
+    for (Emitter &e : emitters_) {
+      if (!e.active) continue;
+      const float rate = 1.0f / e.rate;
+      e.elapsed += dt;
+      e.totalElapsed += dt;
+      if (e.elapsed >= rate) {
+        e.elapsed = 0.0f;
+        const float ax = (e.dx * e.speed) * invDt, ay = (e.dy * e.speed) * invDt;
+        const float dirx = -1.0f * e.dy, diry = 1.0f * e.dx;  // vecmath.h:304-306 Vec2Cross(1, d)
+        const int count = (int)std::floor(e.radius / spacing);
+        if (count > 0) {
+          const float offx = ((dirx * (float)count) * spacing) * 0.5f;
+          const float offy = ((diry * (float)count) * spacing) * 0.5f;
+          std::vector<float> pos(2 * (size_t)count), force(2 * (size_t)count);
+          const float h = params_.kernel_height;
+          for (int q = 0; q < count; ++q) {
+            float px = (dirx * (float)q) * spacing, py = (diry * (float)q) * spacing;
+            px = ((dirx * spacing) * 0.5f) + px;
+            py = ((diry * spacing) * 0.5f) + py;
+            px = (e.px - offx) + px;
+            py = (e.py - offy) + py;
+            const float jx = (splitmix_uniform(e.seed, e.draws) * h) * 0.01f;
+            const float jy = (splitmix_uniform(e.seed, e.draws + 1) * h) * 0.01f;
+            e.draws += 2;
+            pos[2 * q] = jx + px;
+            pos[2 * q + 1] = jy + py;
+            force[2 * q] = ax;
+            force[2 * q + 1] = ay;
+          }
+          addParticles(e.scene, pos.data(), force.data(), (uint64_t)count);
+        }
+      }
+      if (e.totalElapsed >= e.duration) e.active = false;
+    }
+  }
+
+  struct Storage {
+    DevBuf<float2> x0, xp, x1, prev, v, dens, pres, acc;
+    DevBuf<uint32_t> id, id2, keys, keyp, fill, tmpSlot, tmpId, scn, count, startCur, startNext, blockSum, blockOff;
+  };
+
+  nbs_params params_;
+  float gravity_[2] = {0.0f, 0.0f};
+  float ext_[2] = {0.0f, 0.0f};
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  std::vector<HostScene> scenes_;
+  std::vector<DevScene> hostScenes_;
+  uint32_t totalCells_ = 0;
+  bool bodiesDirty_ = true, scenesDirty_ = true;
+  std::vector<Emitter> emitters_;
+
+  Storage d_;
+  Buffers buf_{};
+  DevBuf<DevScene> devScenes_;
+  DevBuf<uint32_t> devSceneBase_;
+  DevBuf<DevBody> devBodies_;
+  DevBuf<float2> devVerts_;
+  uint32_t *stats_ = nullptr;
+  DevBuf<float2> stX_, stPrev_, stV_, stDens_, stPres_;
+  DevBuf<uint32_t> stKey_;
+  DevBuf<int32_t> stScene_;
+  DevBuf<double> diagPartials_, diagOut_;
+
+  uint64_t n_ = 0, nBinned_ = 0, cap_ = 0;
+  bool lists_ = false;
+
+  nbs_stats stats_h_{};
+  bool statsPending_ = false;
+  bool timing_ = false;
+  uint32_t options_ = 0;
+  std::vector<hipEvent_t> events_;
+  double kernelMs_[NBS_KERNEL_COUNT] = {};
+  uint64_t kernelLaunches_[NBS_KERNEL_COUNT] = {};
+
+  hipGraphExec_t graphExec_ = nullptr;
+  DevParams graphParams_{};
+  uint64_t graphN_ = 0;
+};
+
+}  // namespace nbs
+
+struct nbs_sim {
+  std::unique_ptr<nbs::Simulation> impl;
+};
+
+using nbs::Error;
+
+template <typename F>
+static nbs_status guarded(F &&f) {
+  try {
+    nbs::g_lastError.clear();
+    f();
+    return NBS_OK;
+  } catch (const Error &e) {
+    nbs::g_lastError = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    nbs::g_lastError = "host out of memory";
+    return NBS_ERR_OUT_OF_MEMORY;
+  } catch (const std::exception &e) {
+    nbs::g_lastError = e.what();
+    return NBS_ERR_HIP;
+  } catch (...) {
+    nbs::g_lastError = "unknown error";
+    return NBS_ERR_HIP;
+  }
+}
+
+#define NBS_CHECK_SIM(sim) \
+  if (!(sim) || !(sim)->impl) return (nbs::g_lastError = "null handle", NBS_ERR_INVALID_ARGUMENT)
+
+extern "C" {
+
+nbs_status nbs_create(const nbs_params *params, const nbs_domain *domain, uint64_t capacity, nbs_sim **out) {
+  if (!params || !domain || !out) return (nbs::g_lastError = "null argument", NBS_ERR_INVALID_ARGUMENT);
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    nbs::g_lastError = "no HIP device";
+    return NBS_ERR_NO_DEVICE;
+  }
+  return guarded([&] {
+    std::unique_ptr<nbs_sim> s(new nbs_sim);
+    s->impl.reset(new nbs::Simulation(*params, *domain, capacity));
+    *out = s.release();
+  });
+}
+
+nbs_status nbs_destroy(nbs_sim *sim) {
+  if (!sim) return NBS_OK;
+  return guarded([&] { delete sim; });
+}
+
+nbs_status nbs_add_scene(nbs_sim *sim, const nbs_domain *domain, int32_t *scene_out) {
+  NBS_CHECK_SIM(sim);
+  if (!domain) return (nbs::g_lastError = "null domain", NBS_ERR_INVALID_ARGUMENT);
+  return guarded([&] {
+    int s = sim->impl->addScene(*domain);
+    if (scene_out) *scene_out = s;
+  });
+}
+
+nbs_status nbs_get_scene_count(nbs_sim *sim, int32_t *count) {
+  NBS_CHECK_SIM(sim);
+  if (!count) return NBS_ERR_INVALID_ARGUMENT;
+  *count = sim->impl->sceneCount();
+  return NBS_OK;
+}
+
+nbs_status nbs_set_params(nbs_sim *sim, const nbs_params *params) {
+  NBS_CHECK_SIM(sim);
+  if (!params) return NBS_ERR_INVALID_ARGUMENT;
+  return guarded([&] { sim->impl->setParams(*params); });
+}
+
+nbs_status nbs_get_params(nbs_sim *sim, nbs_params *params) {
+  NBS_CHECK_SIM(sim);
+  if (!params) return NBS_ERR_INVALID_ARGUMENT;
+  *params = sim->impl->params();
+  return NBS_OK;
+}
+
+nbs_status nbs_set_gravity(nbs_sim *sim, float gx, float gy) {
+  NBS_CHECK_SIM(sim);
+  sim->impl->setGravity(gx, gy);
+  return NBS_OK;
+}
+
+nbs_status nbs_add_external_force(nbs_sim *sim, float fx, float fy) {
+  NBS_CHECK_SIM(sim);
+  sim->impl->addExternalForce(fx, fy);
+  return NBS_OK;
+}
+
+nbs_status nbs_clear_external_force(nbs_sim *sim) {
+  NBS_CHECK_SIM(sim);
+  sim->impl->clearExternalForce();
+  return NBS_OK;
+}
+
+nbs_status nbs_reset_stats(nbs_sim *sim) {
+  NBS_CHECK_SIM(sim);
+  sim->impl->resetStats();
+  return NBS_OK;
+}
+
+nbs_status nbs_clear_bodies(nbs_sim *sim) {
+  NBS_CHECK_SIM(sim);
+  sim->impl->clearBodies();
+  return NBS_OK;
+}
+
+nbs_status nbs_clear_particles(nbs_sim *sim) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->clearParticles(); });
+}
+
+nbs_status nbs_clear_emitters(nbs_sim *sim) {
+  NBS_CHECK_SIM(sim);
+  sim->impl->clearEmitters();
+  return NBS_OK;
+}
+
+nbs_status nbs_add_plane(nbs_sim *sim, int32_t scene, float nx, float ny, float distance) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->addBody(scene, nbs::HostBody{nbs::BODY_PLANE, {nx, ny, distance, 0.0f}, {}}); });
+}
+
+nbs_status nbs_add_circle(nbs_sim *sim, int32_t scene, float px, float py, float radius) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->addBody(scene, nbs::HostBody{nbs::BODY_CIRCLE, {px, py, radius, 0.0f}, {}}); });
+}
+
+nbs_status nbs_add_line_segment(nbs_sim *sim, int32_t scene, float ax, float ay, float bx, float by) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->addBody(scene, nbs::HostBody{nbs::BODY_SEGMENT, {ax, ay, bx, by}, {}}); });
+}
+
+nbs_status nbs_add_polygon(nbs_sim *sim, int32_t scene, uint64_t vertex_count, const float *verts_xy) {
+  NBS_CHECK_SIM(sim);
+  if (!verts_xy || vertex_count < 3 || vertex_count > 8) {
+    nbs::g_lastError = "a polygon has 3..8 vertices";
+    return NBS_ERR_INVALID_ARGUMENT;
+  }
+  return guarded([&] {
+    nbs::HostBody b{nbs::BODY_POLYGON, {0.0f, 0.0f, 0.0f, 0.0f}, {}};
+    b.verts.assign(verts_xy, verts_xy + 2 * vertex_count);
+    sim->impl->addBody(scene, std::move(b));
+  });
+}
+
+nbs_status nbs_add_particles(nbs_sim *sim, int32_t scene, const float *pos_xy, const float *force_xy, uint64_t n,
+                             uint64_t *first_index) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] {
+    uint64_t f = sim->impl->addParticles(scene, pos_xy, force_xy, n);
+    if (first_index) *first_index = f;
+  });
+}
+
+nbs_status nbs_add_volume(nbs_sim *sim, int32_t scene, float center_x, float center_y, float force_x, float force_y,
+                          int32_t count_x, int32_t count_y, float spacing, uint64_t jitter_seed) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] {
+    sim->impl->addVolume(scene, center_x, center_y, force_x, force_y, count_x, count_y, spacing, jitter_seed);
+  });
+}
+
+nbs_status nbs_add_emitter(nbs_sim *sim, int32_t scene, float pos_x, float pos_y, float dir_x, float dir_y,
+                           float radius, float speed, float rate, float duration, uint64_t jitter_seed) {
+  NBS_CHECK_SIM(sim);
+  if (!(rate > 0.0f)) return (nbs::g_lastError = "rate must be positive", NBS_ERR_INVALID_ARGUMENT);
+  return guarded([&] {
+    nbs::Emitter e{scene, pos_x, pos_y, dir_x, dir_y, radius, speed, rate, duration, 0.0f, 0.0f, true, jitter_seed, 0};
+    sim->impl->addEmitter(e);
+  });
+}
+
+nbs_status nbs_update(nbs_sim *sim, float dt, uint32_t nsteps) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->update(dt, nsteps); });
+}
+
+nbs_status nbs_get_particle_count(nbs_sim *sim, uint64_t *count) {
+  NBS_CHECK_SIM(sim);
+  if (!count) return NBS_ERR_INVALID_ARGUMENT;
+  *count = sim->impl->count();
+  return NBS_OK;
+}
+
+nbs_status nbs_get_state(nbs_sim *sim, float *x, float *prev, float *v, float *density, float *pressure,
+                         uint32_t *cell_key, int32_t *scene) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->getState(x, prev, v, density, pressure, cell_key, scene); });
+}
+
+nbs_status nbs_set_state(nbs_sim *sim, const float *x, const float *v) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->setState(x, v); });
+}
+
+nbs_status nbs_get_stats(nbs_sim *sim, nbs_stats *stats) {
+  NBS_CHECK_SIM(sim);
+  if (!stats) return NBS_ERR_INVALID_ARGUMENT;
+  return guarded([&] { *stats = sim->impl->stats(); });
+}
+
+nbs_status nbs_get_kernel_times(nbs_sim *sim, double *avg_ms, uint64_t *launches) {
+  NBS_CHECK_SIM(sim);
+  sim->impl->kernelTimes(avg_ms, launches);
+  return NBS_OK;
+}
+
+nbs_status nbs_set_timing(nbs_sim *sim, int32_t enabled) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->setTiming(enabled != 0); });
+}
+
+nbs_status nbs_set_options(nbs_sim *sim, uint32_t flags) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->setOptions(flags); });
+}
+
+nbs_status nbs_get_diagnostics(nbs_sim *sim, nbs_diagnostics *out) {
+  NBS_CHECK_SIM(sim);
+  if (!out) return NBS_ERR_INVALID_ARGUMENT;
+  return guarded([&] { *out = sim->impl->diagnostics(); });
+}
+
+nbs_status nbs_get_total_cells(nbs_sim *sim, uint64_t *total_cells) {
+  NBS_CHECK_SIM(sim);
+  if (!total_cells) return NBS_ERR_INVALID_ARGUMENT;
+  *total_cells = sim->impl->totalCells();
+  return NBS_OK;
+}
+
+nbs_status nbs_get_grid(nbs_sim *sim, uint32_t *sorted_ids, uint32_t *cell_start) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->getGrid(sorted_ids, cell_start); });
+}
+
+nbs_status nbs_is_multithreading_supported(nbs_sim *sim, int32_t *supported) {
+  NBS_CHECK_SIM(sim);
+  if (!supported) return NBS_ERR_INVALID_ARGUMENT;
+  *supported = 0;
+  return NBS_OK;
+}
+
+nbs_status nbs_selftest_math(const float *a, const float *b, uint64_t n, float *out) {
+  if (!a || !b || !out) return NBS_ERR_INVALID_ARGUMENT;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return NBS_ERR_NO_DEVICE;
+  return guarded([&] {
+    nbs::DevBuf<float> da, db, dout;
+    da.alloc(n);
+    db.alloc(n);
+    dout.alloc(4 * n);
+    NBS_HIP(hipMemcpy(da.p, a, n * 4, hipMemcpyHostToDevice));
+    NBS_HIP(hipMemcpy(db.p, b, n * 4, hipMemcpyHostToDevice));
+    nbs::launch_selftest_math(da.p, db.p, n, dout.p, nullptr);
+    NBS_HIP(hipGetLastError());
+    NBS_HIP(hipMemcpy(out, dout.p, 16 * n, hipMemcpyDeviceToHost));
+  });
+}
+
+const char *nbs_status_string(int32_t status) {
+  switch (status) {
+    case NBS_OK: return "NBS_OK";
+    case NBS_ERR_INVALID_ARGUMENT: return "NBS_ERR_INVALID_ARGUMENT";
+    case NBS_ERR_OUT_OF_MEMORY: return "NBS_ERR_OUT_OF_MEMORY";
+    case NBS_ERR_HIP: return "NBS_ERR_HIP";
+    case NBS_ERR_CAPACITY: return "NBS_ERR_CAPACITY";
+    case NBS_ERR_NON_FINITE: return "NBS_ERR_NON_FINITE";
+    case NBS_ERR_NO_DEVICE: return "NBS_ERR_NO_DEVICE";
+    case NBS_ERR_UNSUPPORTED: return "NBS_ERR_UNSUPPORTED";
+    default: return "NBS_ERR_UNKNOWN";
+  }
+}
+
+const char *nbs_last_error(void) { return nbs::g_lastError.c_str(); }
+
+}  // extern "C"
