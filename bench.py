@@ -4,13 +4,15 @@ Default workload: 1,048,576 particles (1024x1024 block at 0.5h, seeded
 jitter) in a 600h x 600h box with 4 walls and 64 circles; h=1, rho0=10,
 k=0.004, k_near=0.01, sigma=0, beta=0.3, g=(0,-9.8), dt=0.005.  A step is one
 full SPH step of that scene.  W warm-up steps run untimed; then K steps are
-timed with the inputs resident in HBM.
+timed with the inputs resident in HBM: the warm-up runs on a throwaway
+instance and the timed steps are steps 0..K-1 of a fresh instance.
 
 roofline: the dominant kernel's algorithmic bytes per launch divided by its
 average duration, measured with HIP events on the library's stream over the
 timed steps.  cpu_baseline: the reference's multi-threaded step restated in
 C++ (oracle/, Gauss-Seidel order, pooled phases), timed on this host on a
-bounded sample that starts from the GPU state after warm-up.
+bounded sample that starts from the GPU state in the middle of the timed
+window.
 """
 from __future__ import annotations
 
@@ -80,9 +82,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2"])
     ap.add_argument("--graph", action="store_true", help="replay steps as HIP graphs (no per-kernel timing)")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile-cap", default="auto",
+                    help="LDS tile capacities of the viscosity, density and relaxation kernels, e.g. 2048,2048,1024; "
+                         "'auto' lets the library's autotuner choose")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -95,13 +100,22 @@ def main():
     specs, desc = build_workload(args.workload)
     n = sum(s.n for s in specs)
     log(f"workload {args.workload}: {desc}")
-    sim = ParticleSimulation.from_scene(specs[0], SPHParams()) if len(specs) == 1 else \
-        ParticleSimulation.from_scenes(specs, SPHParams())
-    sim.set_options(hip_graph=args.graph)
-    log(f"warm-up {args.warmup} steps")
-    sim.Update(DT, args.warmup)
-    state_after_warmup = sim.get_state() if not args.no_cpu_baseline else None
 
+    def make_sim():
+        sim = ParticleSimulation.from_scene(specs[0], SPHParams()) if len(specs) == 1 else \
+            ParticleSimulation.from_scenes(specs, SPHParams())
+        sim.set_options(hip_graph=args.graph)
+        if args.tile_cap != "auto":
+            sim.set_tuning(autotune=False, tile_cap=[int(c) for c in args.tile_cap.split(",")])
+        return sim
+
+    # Warm-up on a throwaway instance, then K timed steps of a fresh instance
+    # from the scene's initial state (the inputs are already in HBM).
+    log(f"warm-up {args.warmup} steps")
+    warm = make_sim()
+    warm.Update(DT, args.warmup)
+    warm.close()
+    sim = make_sim()
     sim.set_timing(not args.graph)
     sim.ResetStats()
     log(f"timing {args.steps} steps")
@@ -110,8 +124,10 @@ def main():
     elapsed = time.perf_counter() - t0
     value = n * args.steps / elapsed
     kt = sim.kernel_times()
+    tuning = sim.get_tuning()
     stats = sim.GetStats()
     diag = sim.diagnostics()
+    sim.close()
     log(f"{value:.4g} particle-steps/s, {1e3 * elapsed / args.steps:.4f} ms/step; kernels {kt}")
 
     roofline = None
@@ -134,11 +150,16 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1)
-        log(f"cpu baseline: {threads} threads, {args.cpu_steps} steps")
-        cv, ct = cpu_baseline(state_after_warmup, specs, threads, args.cpu_steps)
+        mid = args.steps // 2
+        mid_sim = make_sim()
+        mid_sim.Update(DT, mid)
+        state = mid_sim.get_state()
+        mid_sim.close()
+        log(f"cpu baseline: {threads} threads, {args.cpu_steps} steps from step {mid}")
+        cv, ct = cpu_baseline(state, specs, threads, args.cpu_steps)
         cpu = {"value": round(cv, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the same scene from the GPU state after {args.warmup} "
-                         f"warm-up steps (+1 untimed step to rebuild lists), {ct:.1f} s",
+               "sample": f"{args.cpu_steps} steps of the same scene from the GPU state at step {mid} (the middle of "
+                         f"the timed window; +1 untimed step to rebuild the neighbour lists), {ct:.1f} s",
                "gpu_over_cpu": round(value / cv, 1)}
         log(f"cpu baseline {cv:.4g} particle-steps/s")
 
@@ -146,7 +167,8 @@ def main():
         "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": desc, "particles": n, "dt": DT, "method": "plain", "hip_graph": args.graph},
+        "config": {"workload": desc, "particles": n, "dt": DT, "method": "plain", "hip_graph": args.graph,
+                   "tuning": tuning},
         "roofline": roofline, "cpu_baseline": cpu,
         "stats": {k: v for k, v in stats.items() if k != "time"},
         "diagnostics_fp64": {k: diag[k] for k in ("mean_density", "max_density", "mean_abs_density_error",

@@ -58,6 +58,10 @@ class NbsDiagnostics(ctypes.Structure):
                 ("rms_density_error", c_double), ("centroid", c_double * 2), ("non_finite_count", c_uint64)]
 
 
+class NbsTuning(ctypes.Structure):
+    _fields_ = [("autotune", c_int32), ("retune_interval", c_uint32), ("tile_cap", c_int32 * 3)]
+
+
 class NbsError(RuntimeError):
     def __init__(self, code: int, func: str, message: str):
         self.code = code
@@ -97,6 +101,8 @@ SIGNATURES = {
     "nbs_set_timing": [vp, c_int32],
     "nbs_set_options": [vp, c_uint32],
     "nbs_get_diagnostics": [vp, P(NbsDiagnostics)],
+    "nbs_set_tuning": [vp, P(NbsTuning)],
+    "nbs_get_tuning": [vp, P(NbsTuning)],
     "nbs_get_total_cells": [vp, P(c_uint64)],
     "nbs_get_grid": [vp, vp, vp],
     "nbs_is_multithreading_supported": [vp, P(c_int32)],

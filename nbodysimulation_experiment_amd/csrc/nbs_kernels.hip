@@ -209,19 +209,150 @@ __device__ __forceinline__ RowRange row_range(const uint32_t *start, const DevSc
   return {start[row + xlo], start[row + xhi + 1]};
 }
 
+// LDS tile of a workgroup.  The workgroup's particles occupy consecutive
+// sorted slots, so their keys span [kA, kB] and the candidates of all of them
+// lie in three slot ranges, one per row offset dy: the cells with keys in
+// [kA + dy*nx - 1, kB + dy*nx + 1] (clamped to the scene).  Each range is
+// staged at tile index off[d]; a slot j of range d sits at off[d] + j - rowStart[d].
+struct TilePlan {
+  uint32_t rowStart[3];
+  uint32_t off[4];  // off[3] = staged particles
+  int use;          // 1 when the ranges fit the tile
+};
+
+template <int CAP>
+__device__ __forceinline__ void plan_tile(const uint32_t *start, const DevScene &sc, uint32_t kA, uint32_t kB,
+                                          TilePlan &t) {
+  const int64_t first = sc.cellBase, last = (int64_t)sc.cellBase + (int64_t)sc.nx * sc.ny - 1;
+  uint32_t total = 0;
+  for (int d = 0; d < 3; ++d) {
+    const int64_t lo = max((int64_t)kA + (int64_t)(d - 1) * sc.nx - 1, first);
+    const int64_t hi = min((int64_t)kB + (int64_t)(d - 1) * sc.nx + 1, last);
+    uint32_t a = 0, e = 0;
+    if (lo <= hi) {
+      a = start[lo];
+      e = start[hi + 1];
+    }
+    t.rowStart[d] = a;
+    t.off[d] = total;
+    total += e - a;
+  }
+  t.off[3] = total;
+  t.use = (CAP > 0 && total <= (uint32_t)CAP) ? 1 : 0;
+}
+
+// Cooperative, coalesced copy of the planned ranges of src into the tile.
+__device__ __forceinline__ void stage(const TilePlan &t, const float2 *__restrict__ src, float2 *dst) {
+  for (uint32_t q = threadIdx.x; q < t.off[3]; q += kBlock) {
+    const int d = (q >= t.off[1] ? 1 : 0) + (q >= t.off[2] ? 1 : 0);
+    dst[q] = src[t.rowStart[d] + (q - t.off[d])];
+  }
+}
+
+// Index of slot j of row range d in the neighbour data source.
+struct TileIndex {
+  uint32_t rowStart[3], off[3];
+  __device__ __forceinline__ explicit TileIndex(const TilePlan &t) {
+    for (int d = 0; d < 3; ++d) rowStart[d] = t.rowStart[d], off[d] = t.off[d];
+  }
+  __device__ __forceinline__ uint32_t operator()(int d, uint32_t j) const { return off[d] + (j - rowStart[d]); }
+};
+struct SlotIndex {
+  __device__ __forceinline__ uint32_t operator()(int, uint32_t j) const { return j; }
+};
+
+// Calls f(q, Rx, Ry, r2) for every candidate of the particle at xi in cell
+// (cx, cy) whose r2 < h2 (sph.h:470, 488, 502), in candidate order, where q
+// indexes the candidate in the data source (tile or slot arrays); returns the
+// number of candidates (the reference's neighbour count).  Each row range is
+// taken in chunks of 64: a cheap pass marks the in-kernel candidates in a
+// 64-bit mask, then the pair work runs only for the marked ones, lowest first,
+// so the accumulation order is the candidate order and the expensive work is
+// not issued for candidates outside the kernel radius.
+template <typename Idx, typename F>
+__device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const DevScene &sc, int cx, int cy,
+                                                  const float2 *X, Idx idx, float2 xi, float h2, F &&f) {
+  uint32_t cand = 0;
+  for (int d = 0; d < 3; ++d) {
+    const RowRange rr = row_range(start, sc, cx, cy, d - 1);
+    cand += rr.j1 - rr.j0;
+    for (uint32_t base = rr.j0; base < rr.j1; base += 64u) {
+      const uint32_t cnt = min(64u, rr.j1 - base);
+      const uint32_t q0 = idx(d, base);
+      uint64_t mask = 0;
+#pragma unroll 4
+      for (uint32_t t = 0; t < cnt; ++t) {
+        const float2 xj = X[q0 + t];
+        const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;
+        const float r2 = Rx * Rx + Ry * Ry;
+        mask |= (uint64_t)(r2 < h2) << t;
+      }
+      while (mask) {
+        const uint32_t t = (uint32_t)__builtin_ctzll(mask);
+        mask &= mask - 1u;
+        const uint32_t q = q0 + t;
+        const float2 xj = X[q];
+        const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;
+        f(q, Rx, Ry, Rx * Rx + Ry * Ry);
+      }
+    }
+  }
+  return cand;
+}
+
 // ---------------------------------------------------------------------------
 // 1. integrate + viscosity + predict + key + count
 // ---------------------------------------------------------------------------
-template <bool MULTI>
+// demo4.cpp:223-237 in Jacobi order: both halves of each pair's impulse from
+// the integrated velocities (neighbours have no own acceleration: v_j + g dt).
+template <typename Idx>
+__device__ __forceinline__ void viscosity_sum(const uint32_t *start, const DevScene &sc, int cx, int cy,
+                                              const float2 *X, const float2 *V, Idx idx, const DevParams &p,
+                                              float2 xi, float vx, float vy, float &accx, float &accy) {
+  for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t q, float Rx, float Ry, float r2) {
+    const float2 vj = V[q];
+    const float vjx = p.gdtx + vj.x, vjy = p.gdty + vj.y;
+    const float r = sqrtf(r2);                            // sph.h:503
+    const float qq = r * p.invH;                          // sph.h:504
+    const float l = (r == 0.0f) ? 1.0f : r;               // sph.h:505 Vec2Normalize
+    const float inv = 1.0f / l;
+    const float nX = Rx * inv, nY = Ry * inv;
+    const float u = (vx - vjx) * nX + (vy - vjy) * nY;    // sph.h:506
+    if (u > 0.0f) {
+      const float f = (1.0f - qq) * (p.sigma * u + p.beta * (u * u));  // sph.h:508
+      const float ix = ((f * nX) * 0.5f) * p.dt;         // demo4.cpp:233 force * 0.5f * deltaTime
+      const float iy = ((f * nY) * 0.5f) * p.dt;
+      accx = accx - ix;                                   // own visit
+      accy = accy - iy;
+      accx = accx - ix;                                   // neighbour's visit (f_ji = -f_ij)
+      accy = accy - iy;
+    }
+  });
+}
+
+template <bool MULTI, int CAP>
 __global__ __launch_bounds__(kBlock) void k_visc_predict(Buffers b, DevParams p, DevScene s0, int n, int nBinned,
                                                          int lists) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    b.stats[0] = 0xFFFFFFFFu;
-    b.stats[1] = 0u;
-    b.stats[2] = 0xFFFFFFFFu;
-    b.stats[3] = 0u;
+  __shared__ float2 tx[CAP > 0 ? CAP : 1], tv[CAP > 0 ? CAP : 1];
+  __shared__ TilePlan plan;
+  const int i0 = (int)(xcd_block() * kBlock);
+  const int i = i0 + (int)threadIdx.x;
+  // Viscosity runs for slots of the previous binning when lists exist.
+  const bool tileVisc = lists && i0 < nBinned;
+  if (tileVisc) {
+    if (threadIdx.x == 0) {
+      const uint32_t kA = b.keys[i0], kB = b.keys[min(i0 + kBlock, nBinned) - 1];
+      const DevScene sa = scene_of_slot<MULTI>(b, s0, kA);
+      plan_tile<CAP>(b.startCur, sa, kA, kB, plan);
+      if (MULTI && sa.cellBase != scene_of_slot<MULTI>(b, s0, kB).cellBase) plan.use = 0;
+    }
+    __syncthreads();
+    if (CAP > 0 && plan.use) {
+      stage(plan, b.x0, tx);
+      stage(plan, b.v, tv);
+    }
+    __syncthreads();
   }
-  const int i = (int)(xcd_block() * kBlock + threadIdx.x);
   if (i >= n) return;
   const float2 xi = b.x0[i];
   const float2 vi = b.v[i];
@@ -242,39 +373,14 @@ __global__ __launch_bounds__(kBlock) void k_visc_predict(Buffers b, DevParams p,
   float vy = (aY * p.dt) + vi.y;
 
   if (lists && binned) {
-    // demo4.cpp:223-237 in Jacobi order: both halves of each pair's impulse,
-    // computed from the integrated velocities of the previous binning's slots.
     const uint32_t local = b.keys[i] - sc.cellBase;
     const int cy = (int)(local / (uint32_t)sc.nx);
     const int cx = (int)local - cy * sc.nx;
     float accx = 0.0f, accy = 0.0f;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const RowRange rr = row_range(b.startCur, sc, cx, cy, dy);
-      for (uint32_t j = rr.j0; j < rr.j1; ++j) {
-        const float2 xj = b.x0[j];
-        const float2 vj = b.v[j];
-        const float vjx = p.gdtx + vj.x, vjy = p.gdty + vj.y;
-        const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;  // sph.h:500
-        const float r2 = Rx * Rx + Ry * Ry;               // sph.h:501
-        if (r2 < p.h2) {                                  // sph.h:502
-          const float r = sqrtf(r2);                      // sph.h:503
-          const float q = r * p.invH;                     // sph.h:504
-          const float l = (r == 0.0f) ? 1.0f : r;         // sph.h:505 Vec2Normalize
-          const float inv = 1.0f / l;
-          const float nX = Rx * inv, nY = Ry * inv;
-          const float u = (vx - vjx) * nX + (vy - vjy) * nY;  // sph.h:506
-          if (u > 0.0f) {
-            const float f = (1.0f - q) * (p.sigma * u + p.beta * (u * u));  // sph.h:508
-            const float ix = ((f * nX) * 0.5f) * p.dt;   // demo4.cpp:233 force * 0.5f * deltaTime
-            const float iy = ((f * nY) * 0.5f) * p.dt;
-            accx = accx - ix;                             // own visit
-            accy = accy - iy;
-            accx = accx - ix;                             // neighbour's visit (f_ji = -f_ij)
-            accy = accy - iy;
-          }
-        }
-      }
-    }
+    if (CAP > 0 && plan.use)
+      viscosity_sum(b.startCur, sc, cx, cy, tx, tv, TileIndex(plan), p, xi, vx, vy, accx, accy);
+    else
+      viscosity_sum(b.startCur, sc, cx, cy, b.x0, b.v, SlotIndex(), p, xi, vx, vy, accx, accy);
     vx = accx + vx;
     vy = accy + vy;
   }
@@ -331,6 +437,29 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *total)
   return off + incl - v;
 }
 
+// Workgroup-wide min and max (result valid in every thread).
+__device__ __forceinline__ void block_min_max(uint32_t &mn, uint32_t &mx) {
+  __shared__ uint32_t shMin[kBlock / 64], shMax[kBlock / 64];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
+    mn = min(mn, (uint32_t)__shfl_xor((int)mn, d, 64));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    shMin[wave] = mn;
+    shMax[wave] = mx;
+  }
+  __syncthreads();
+  mn = shMin[0];
+  mx = shMax[0];
+#pragma unroll
+  for (int w = 1; w < kBlock / 64; ++w) {
+    mn = min(mn, shMin[w]);
+    mx = max(mx, shMax[w]);
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(Buffers b, uint32_t cells) {
   const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanItems;
   uint32_t s = 0, mx = 0, mn = 0xFFFFFFFFu;
@@ -344,15 +473,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(Buffers b, uint32_t cell
   }
   uint32_t tot;
   block_excl_scan(s, &tot);
-  // occupied-cell min / max
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
-    mn = min(mn, (uint32_t)__shfl_xor((int)mn, d, 64));
-  }
-  if ((threadIdx.x & 63) == 0) {
-    if (mx > 0u) atomicMax(&b.stats[3], mx);
-    if (mn != 0xFFFFFFFFu) atomicMin(&b.stats[2], mn);
+  // occupied-cell min / max, one pair per workgroup
+  block_min_max(mn, mx);
+  if (threadIdx.x == 0) {
+    b.cellStats[2 * blockIdx.x] = mn;
+    b.cellStats[2 * blockIdx.x + 1] = mx;
   }
   if (threadIdx.x == 0) b.blockSum[blockIdx.x] = tot;
 }
@@ -425,9 +550,39 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
 // ---------------------------------------------------------------------------
 // 4. density + pressure
 // ---------------------------------------------------------------------------
-template <bool MULTI>
+template <typename Idx>
+__device__ __forceinline__ uint32_t density_sum(const uint32_t *start, const DevScene &sc, int cx, int cy,
+                                                const float2 *X, Idx idx, const DevParams &p, float2 xi, float &d0,
+                                                float &d1) {
+  return for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t, float, float, float r2) {
+    const float r = sqrtf(r2);                            // sph.h:471
+    const float term = 1.0f - r * p.invH;                 // sph.h:472
+    d0 = d0 + (term * term);                              // sph.h:473
+    d1 = d1 + ((term * term) * term);                     // sph.h:474
+  });
+}
+
+// Plans the tile of the workgroup at slots [i0, i0 + kBlock) of the new binning.
+template <bool MULTI, int CAP>
+__device__ __forceinline__ void plan_new(const Buffers &b, const DevScene &s0, int i0, int n, TilePlan &plan) {
+  if (threadIdx.x == 0) {
+    const uint32_t kA = b.keys[i0], kB = b.keys[min(i0 + kBlock, n) - 1];
+    const DevScene sa = scene_of_slot<MULTI>(b, s0, kA);
+    plan_tile<CAP>(b.startNext, sa, kA, kB, plan);
+    if (MULTI && sa.cellBase != scene_of_slot<MULTI>(b, s0, kB).cellBase) plan.use = 0;
+  }
+  __syncthreads();
+}
+
+template <bool MULTI, int CAP>
 __global__ __launch_bounds__(kBlock) void k_density(Buffers b, DevParams p, DevScene s0, int n) {
-  const int i = (int)(xcd_block() * kBlock + threadIdx.x);
+  __shared__ float2 tx[CAP > 0 ? CAP : 1];
+  __shared__ TilePlan plan;
+  const int i0 = (int)(xcd_block() * kBlock);
+  const int i = i0 + (int)threadIdx.x;
+  plan_new<MULTI, CAP>(b, s0, i0, n, plan);
+  if (CAP > 0 && plan.use) stage(plan, b.x1, tx);
+  __syncthreads();
   uint32_t cand = 0;
   if (i < n) {
     const uint32_t key = b.keys[i];
@@ -437,43 +592,61 @@ __global__ __launch_bounds__(kBlock) void k_density(Buffers b, DevParams p, DevS
     const int cx = (int)local - cy * sc.nx;
     const float2 xi = b.x1[i];
     float d0 = 0.0f, d1 = 0.0f;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const RowRange rr = row_range(b.startNext, sc, cx, cy, dy);
-      cand += rr.j1 - rr.j0;
-      for (uint32_t j = rr.j0; j < rr.j1; ++j) {
-        const float2 xj = b.x1[j];
-        const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;  // sph.h:466
-        const float r2 = Rx * Rx + Ry * Ry;               // sph.h:467
-        if (r2 < p.h2) {                                  // sph.h:470
-          const float r = sqrtf(r2);                      // sph.h:471
-          const float term = 1.0f - r * p.invH;           // sph.h:472
-          d0 = d0 + (term * term);                        // sph.h:473
-          d1 = d1 + ((term * term) * term);               // sph.h:474
-        }
-      }
-    }
+    if (CAP > 0 && plan.use)
+      cand = density_sum(b.startNext, sc, cx, cy, tx, TileIndex(plan), p, xi, d0, d1);
+    else
+      cand = density_sum(b.startNext, sc, cx, cy, b.x1, SlotIndex(), p, xi, d0, d1);
     b.dens[i] = make_float2(d0, d1);
     b.pres[i] = make_float2(p.k * (d0 - p.rho0), p.kn * d1);  // sph.h:478-481
   }
-  // candidate-count statistics (demo4.cpp:369-376)
+  // candidate-count statistics (demo4.cpp:369-376): one min/max pair per
+  // workgroup, reduced when the statistics are read.
   uint32_t mn = (i < n) ? cand : 0xFFFFFFFFu, mx = cand;
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
-    mn = min(mn, (uint32_t)__shfl_xor((int)mn, d, 64));
-  }
-  if ((threadIdx.x & 63) == 0 && mn != 0xFFFFFFFFu) {
-    atomicMin(&b.stats[0], mn);
-    atomicMax(&b.stats[1], mx);
+  block_min_max(mn, mx);
+  if (threadIdx.x == 0) {
+    b.candStats[2 * blockIdx.x] = mn;
+    b.candStats[2 * blockIdx.x + 1] = mx;
   }
 }
 
 // ---------------------------------------------------------------------------
 // 5. relaxation + collisions + velocity
 // ---------------------------------------------------------------------------
-template <bool MULTI>
+// demo4.cpp:239-255 in Jacobi order: -delta_ij/2 from the particle's own visit
+// and +delta_ji/2 from the neighbour's visit (n_ji = -n_ij).
+template <typename Idx>
+__device__ __forceinline__ void delta_sum(const uint32_t *start, const DevScene &sc, int cx, int cy, const float2 *X,
+                                          const float2 *P, Idx idx, const DevParams &p, float2 xi, float2 pi,
+                                          float &accx, float &accy) {
+  for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t q, float Rx, float Ry, float r2) {
+    const float2 pj = P[q];
+    const float r = sqrtf(r2);                            // sph.h:489
+    const float l = (r == 0.0f) ? 1.0f : r;               // sph.h:490 Vec2Normalize
+    const float inv = 1.0f / l;
+    const float nX = Rx * inv, nY = Ry * inv;
+    const float term = 1.0f - r * p.invH;                 // sph.h:491
+    const float tt = term * term;
+    const float dij = p.dt2 * (pi.x * term + pi.y * tt);  // sph.h:492 with P_i
+    const float dji = p.dt2 * (pj.x * term + pj.y * tt);  // sph.h:492 with P_j (j's visit)
+    accx = accx - ((dij * nX) * 0.5f);                    // demo4.cpp:251 dx -= delta * 0.5f
+    accy = accy - ((dij * nY) * 0.5f);
+    accx = accx - ((dji * nX) * 0.5f);                    // demo4.cpp:250 (j's visit)
+    accy = accy - ((dji * nY) * 0.5f);
+  });
+}
+
+template <bool MULTI, int CAP>
 __global__ __launch_bounds__(kBlock) void k_delta(Buffers b, DevParams p, DevScene s0, int n) {
-  const int i = (int)(xcd_block() * kBlock + threadIdx.x);
+  __shared__ float2 tx[CAP > 0 ? CAP : 1], tp[CAP > 0 ? CAP : 1];
+  __shared__ TilePlan plan;
+  const int i0 = (int)(xcd_block() * kBlock);
+  const int i = i0 + (int)threadIdx.x;
+  plan_new<MULTI, CAP>(b, s0, i0, n, plan);
+  if (CAP > 0 && plan.use) {
+    stage(plan, b.x1, tx);
+    stage(plan, b.pres, tp);
+  }
+  __syncthreads();
   if (i >= n) return;
   const uint32_t key = b.keys[i];
   const DevScene sc = scene_of_slot<MULTI>(b, s0, key);
@@ -483,31 +656,10 @@ __global__ __launch_bounds__(kBlock) void k_delta(Buffers b, DevParams p, DevSce
   const float2 xi = b.x1[i];
   const float2 pi = b.pres[i];
   float accx = 0.0f, accy = 0.0f;
-  for (int dy = -1; dy <= 1; ++dy) {
-    const RowRange rr = row_range(b.startNext, sc, cx, cy, dy);
-    for (uint32_t j = rr.j0; j < rr.j1; ++j) {
-      const float2 xj = b.x1[j];
-      const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;    // sph.h:486
-      const float r2 = Rx * Rx + Ry * Ry;                 // sph.h:487
-      if (r2 < p.h2) {                                    // sph.h:488
-        const float2 pj = b.pres[j];
-        const float r = sqrtf(r2);                        // sph.h:489
-        const float l = (r == 0.0f) ? 1.0f : r;           // sph.h:490 Vec2Normalize
-        const float inv = 1.0f / l;
-        const float nX = Rx * inv, nY = Ry * inv;
-        const float term = 1.0f - r * p.invH;             // sph.h:491
-        const float tt = term * term;
-        const float dij = p.dt2 * (pi.x * term + pi.y * tt);  // sph.h:492 with P_i
-        const float dji = p.dt2 * (pj.x * term + pj.y * tt);  // sph.h:492 with P_j (j's visit)
-        // demo4.cpp:251 dx -= delta * 0.5f (own visit)
-        accx = accx - ((dij * nX) * 0.5f);
-        accy = accy - ((dij * nY) * 0.5f);
-        // demo4.cpp:250 neighbour += delta * 0.5f (j's visit; n_ji = -n_ij)
-        accx = accx - ((dji * nX) * 0.5f);
-        accy = accy - ((dji * nY) * 0.5f);
-      }
-    }
-  }
+  if (CAP > 0 && plan.use)
+    delta_sum(b.startNext, sc, cx, cy, tx, tp, TileIndex(plan), p, xi, pi, accx, accy);
+  else
+    delta_sum(b.startNext, sc, cx, cy, b.x1, b.pres, SlotIndex(), p, xi, pi, accx, accy);
   float x = accx + xi.x;                                  // demo4.cpp:253
   float y = accy + xi.y;
   collide(b, p, sc, x, y);                                // demo4.cpp:411-444
@@ -645,13 +797,24 @@ void launch_unbin(const Buffers &b, int nBinned, hipStream_t s) {
   hipLaunchKernelGGL(k_unbin, dim3(grid_for(nBinned)), dim3(kBlock), 0, s, b, nBinned);
 }
 
-void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int nBinned, int lists,
-                         hipStream_t s) {
-  if (n <= 0) return;
+template <int CAP>
+static void visc_cap(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int nBinned, int lists,
+                     hipStream_t s) {
   if (b.nScenes > 1)
-    hipLaunchKernelGGL(k_visc_predict<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n, nBinned, lists);
+    hipLaunchKernelGGL((k_visc_predict<true, CAP>), dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n, nBinned, lists);
   else
-    hipLaunchKernelGGL(k_visc_predict<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n, nBinned, lists);
+    hipLaunchKernelGGL((k_visc_predict<false, CAP>), dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n, nBinned,
+                       lists);
+}
+
+void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int nBinned, int lists,
+                         int cap, hipStream_t s) {
+  if (n <= 0) return;
+  switch (cap) {
+    case 0: visc_cap<0>(b, p, s0, n, nBinned, lists, s); break;
+    case 1024: visc_cap<1024>(b, p, s0, n, nBinned, lists, s); break;
+    default: visc_cap<2048>(b, p, s0, n, nBinned, lists, s); break;
+  }
 }
 
 void launch_scan(const Buffers &b, uint32_t cells, hipStream_t s) {
@@ -667,20 +830,38 @@ void launch_scatter_reorder(const Buffers &b, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_rank_reorder, dim3(grid_for(n)), dim3(kBlock), 0, s, b, n);
 }
 
-void launch_density(const Buffers &b, const DevParams &p, const DevScene &s0, int n, hipStream_t s) {
-  if (n <= 0) return;
+template <int CAP>
+static void density_cap(const Buffers &b, const DevParams &p, const DevScene &s0, int n, hipStream_t s) {
   if (b.nScenes > 1)
-    hipLaunchKernelGGL(k_density<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n);
+    hipLaunchKernelGGL((k_density<true, CAP>), dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n);
   else
-    hipLaunchKernelGGL(k_density<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n);
+    hipLaunchKernelGGL((k_density<false, CAP>), dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n);
 }
 
-void launch_delta(const Buffers &b, const DevParams &p, const DevScene &s0, int n, hipStream_t s) {
+void launch_density(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int cap, hipStream_t s) {
   if (n <= 0) return;
+  switch (cap) {
+    case 0: density_cap<0>(b, p, s0, n, s); break;
+    case 1024: density_cap<1024>(b, p, s0, n, s); break;
+    default: density_cap<2048>(b, p, s0, n, s); break;
+  }
+}
+
+template <int CAP>
+static void delta_cap(const Buffers &b, const DevParams &p, const DevScene &s0, int n, hipStream_t s) {
   if (b.nScenes > 1)
-    hipLaunchKernelGGL(k_delta<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n);
+    hipLaunchKernelGGL((k_delta<true, CAP>), dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n);
   else
-    hipLaunchKernelGGL(k_delta<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n);
+    hipLaunchKernelGGL((k_delta<false, CAP>), dim3(grid_for(n)), dim3(kBlock), 0, s, b, p, s0, n);
+}
+
+void launch_delta(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int cap, hipStream_t s) {
+  if (n <= 0) return;
+  switch (cap) {
+    case 0: delta_cap<0>(b, p, s0, n, s); break;
+    case 1024: delta_cap<1024>(b, p, s0, n, s); break;
+    default: delta_cap<2048>(b, p, s0, n, s); break;
+  }
 }
 
 void launch_gather_state(const Buffers &b, const DevScene &s0, int n, int nBinned, float2 *x, float2 *prev, float2 *v,

@@ -97,7 +97,6 @@ class Simulation {
     NBS_HIP(hipGetDevice(&dev));
     device_ = dev;
     NBS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-    NBS_HIP(hipMalloc((void **)&stats_, 4 * sizeof(uint32_t)));
     resetStats();
     addScene(d);
     reserve(std::max<uint64_t>(capacity, 1024));
@@ -105,7 +104,6 @@ class Simulation {
   ~Simulation() {
     destroyGraph();
     for (hipEvent_t e : events_) (void)hipEventDestroy(e);
-    if (stats_) (void)hipFree(stats_);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -301,16 +299,24 @@ class Simulation {
     stats_h_.min_cell_particle_count = 500;
     for (int k = 0; k < NBS_KERNEL_COUNT; ++k) kernelMs_[k] = 0.0, kernelLaunches_[k] = 0;
   }
+  // Reduces the per-workgroup minima and maxima the last step left behind.
   nbs_stats stats() {
-    if (statsPending_) {
-      uint32_t s[4];
-      NBS_HIP(hipMemcpy(s, stats_, sizeof(s), hipMemcpyDeviceToHost));
-      stats_h_.min_particle_neighbor_count = s[0];
-      stats_h_.max_particle_neighbor_count = s[1];
-      stats_h_.min_cell_particle_count = (s[2] == 0xFFFFFFFFu) ? 0 : s[2];
-      stats_h_.max_cell_particle_count = s[3];
-      statsPending_ = false;
+    if (statsPending_ && n_ > 0) {
+      const size_t nc = 2 * (size_t)((n_ + kBlock - 1) / kBlock);
+      const size_t ns = 2 * (size_t)((totalCells_ + kScanChunk - 1) / kScanChunk);
+      std::vector<uint32_t> c(nc), s(ns);
+      NBS_HIP(hipMemcpyAsync(c.data(), d_.candStats.p, nc * 4, hipMemcpyDeviceToHost, stream_));
+      NBS_HIP(hipMemcpyAsync(s.data(), d_.cellStats.p, ns * 4, hipMemcpyDeviceToHost, stream_));
+      NBS_HIP(hipStreamSynchronize(stream_));
+      uint32_t cmn = 0xFFFFFFFFu, cmx = 0, smn = 0xFFFFFFFFu, smx = 0;
+      for (size_t k = 0; k < nc; k += 2) cmn = std::min(cmn, c[k]), cmx = std::max(cmx, c[k + 1]);
+      for (size_t k = 0; k < ns; k += 2) smn = std::min(smn, s[k]), smx = std::max(smx, s[k + 1]);
+      stats_h_.min_particle_neighbor_count = cmn;
+      stats_h_.max_particle_neighbor_count = cmx;
+      stats_h_.min_cell_particle_count = (smn == 0xFFFFFFFFu) ? 0 : smn;
+      stats_h_.max_cell_particle_count = smx;
     }
+    statsPending_ = false;
     return stats_h_;
   }
   void kernelTimes(double *avg, uint64_t *launches) const {
@@ -359,12 +365,16 @@ class Simulation {
   void update(float dt, uint32_t nsteps) {
     if (!(dt > 0.0f) || !std::isfinite(dt)) throw Error(NBS_ERR_INVALID_ARGUMENT, "dt must be positive and finite");
     syncBodies();
-    if (timing_) ensureEvents(nsteps);
-    uint32_t s = 0;
+    uint32_t s = 0, slots = 0;  // slots: steps with events this call
     while (s < nsteps) {
       if (!emitters_.empty()) runEmitters(dt);  // demo4.cpp:290-299
       if (n_ == 0) {
         ++s;
+        continue;
+      }
+      if (wantTune() && nsteps - s >= kTuneSteps) {
+        s += tune(dt, slots);
+        slots += kTuneSteps;
         continue;
       }
       const bool graphable = (options_ & NBS_OPT_HIP_GRAPH) && !timing_ && emitters_.empty() && lists_ &&
@@ -372,28 +382,50 @@ class Simulation {
       if (graphable) {
         // Replay pairs of steps (the cell tables and the slot-id arrays
         // alternate between two buffers every step).
-        const uint32_t pairs = (nsteps - s) / 2;
+        uint32_t pairs = (nsteps - s) / 2;
+        if (tuning_.autotune && tuning_.retune_interval > 0 && n_ >= kTuneMinParticles) {
+          // stop replaying when the next re-tuning is due
+          const uint64_t until = tuning_.retune_interval > stepsSinceTune_ ? tuning_.retune_interval - stepsSinceTune_ : 0;
+          pairs = std::min<uint32_t>(pairs, (uint32_t)std::max<uint64_t>(1, until / 2));
+        }
         if (pairs > 0) {
           ensureGraph(dt);
           for (uint32_t q = 0; q < pairs; ++q) NBS_HIP(hipGraphLaunch(graphExec_, stream_));
           s += 2 * pairs;
-          statsPending_ = true;
+          stepsSinceTune_ += 2 * pairs;
           continue;
         }
       }
-      enqueueStep(dt, timing_ ? (int)s : -1);
+      int ev = -1;
+      if (timing_) {
+        ensureEvents(slots + 1);
+        ev = (int)slots++;
+      }
+      enqueueStep(dt, ev);
       swapAfterStep();
+      ++stepsSinceTune_;
       ++s;
     }
     NBS_HIP(hipGetLastError());
     NBS_HIP(hipStreamSynchronize(stream_));
-    if (timing_ && nsteps > 0) collectTimes(nsteps);
+    if (timing_ && slots > 0) collectTimes(slots);
     statsPending_ = true;
     if (options_ & NBS_OPT_CHECK_FINITE) {
       nbs_diagnostics d = diagnostics();
       if (d.non_finite_count) throw Error(NBS_ERR_NON_FINITE, "particle state holds NaN or Inf");
     }
   }
+
+  void setTuning(const nbs_tuning &t) {
+    for (int k = 0; k < 3; ++k)
+      if (t.tile_cap[k] != 0 && t.tile_cap[k] != 1024 && t.tile_cap[k] != 2048)
+        throw Error(NBS_ERR_INVALID_ARGUMENT, "tile capacities are 0, 1024 or 2048");
+    tuning_ = t;
+    tuned_ = false;
+    stepsSinceTune_ = 0;
+    destroyGraph();
+  }
+  nbs_tuning tuning() const { return tuning_; }
 
   hipStream_t stream() const { return stream_; }
 
@@ -475,6 +507,7 @@ class Simulation {
     grow4(d_.id); grow4(d_.keys); grow4(d_.scn);
     d_.xp.alloc(cap); d_.x1.alloc(cap); d_.id2.alloc(cap); d_.keyp.alloc(cap); d_.fill.alloc(cap);
     d_.tmpSlot.alloc(cap); d_.tmpId.alloc(cap);
+    d_.candStats.alloc(2 * ((cap + kBlock - 1) / kBlock));
     cap_ = cap;
     rebindBuffers();
     destroyGraph();
@@ -487,6 +520,7 @@ class Simulation {
     d_.startNext.alloc(c + 1);
     d_.blockSum.alloc(std::max<size_t>(nb, 1));
     d_.blockOff.alloc(std::max<size_t>(nb, 1));
+    d_.cellStats.alloc(2 * std::max<size_t>(nb, 1));
     launch_fill_u32(d_.count.p, 0u, c, stream_);
     launch_fill_u32(d_.startCur.p, 0u, c + 1, stream_);
     launch_fill_u32(d_.startNext.p, 0u, c + 1, stream_);
@@ -518,7 +552,8 @@ class Simulation {
     buf_.id = d_.id.p; buf_.id2 = d_.id2.p; buf_.keys = d_.keys.p; buf_.keyp = d_.keyp.p; buf_.fill = d_.fill.p;
     buf_.tmpSlot = d_.tmpSlot.p; buf_.tmpId = d_.tmpId.p; buf_.scn = d_.scn.p;
     buf_.count = d_.count.p; buf_.startCur = d_.startCur.p; buf_.startNext = d_.startNext.p;
-    buf_.blockSum = d_.blockSum.p; buf_.blockOff = d_.blockOff.p; buf_.stats = stats_;
+    buf_.blockSum = d_.blockSum.p; buf_.blockOff = d_.blockOff.p;
+    buf_.candStats = d_.candStats.p; buf_.cellStats = d_.cellStats.p;
     buf_.scenes = devScenes_.p; buf_.sceneBase = devSceneBase_.p; buf_.nScenes = (int)scenes_.size();
     buf_.bodies = devBodies_.p; buf_.verts = devVerts_.p;
   }
@@ -567,17 +602,64 @@ class Simulation {
     const DevParams p = devParams(dt);
     const DevScene s0 = hostScenes_[0];
     const int n = (int)n_;
+    const int *cap = tuning_.tile_cap;
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 0], stream_));
-    launch_visc_predict(buf_, p, s0, n, (int)nBinned_, lists_ ? 1 : 0, stream_);
+    launch_visc_predict(buf_, p, s0, n, (int)nBinned_, lists_ ? 1 : 0, cap[0], stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 1], stream_));
     launch_scan(buf_, totalCells_, stream_);
     launch_scatter_reorder(buf_, n, stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 2], stream_));
-    launch_density(buf_, p, s0, n, stream_);
+    launch_density(buf_, p, s0, n, cap[1], stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 3], stream_));
-    launch_delta(buf_, p, s0, n, stream_);
+    launch_delta(buf_, p, s0, n, cap[2], stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 4], stream_));
   }
+
+  // Autotuner: every tile capacity gives bit-identical results, so the
+  // candidates are timed on real steps.  Each of kTuneReps rounds runs one step
+  // per capacity (all pair kernels at that capacity) with HIP events; each
+  // kernel then keeps its fastest capacity (minimum over rounds).
+  static constexpr uint32_t kTuneReps = 2;
+  static constexpr uint32_t kTuneSteps = kTuneReps * 3;
+  static constexpr uint64_t kTuneMinParticles = 32768;
+  bool wantTune() const {
+    if (!tuning_.autotune || n_ < kTuneMinParticles || !lists_ || nBinned_ != n_) return false;
+    return !tuned_ || (tuning_.retune_interval > 0 && stepsSinceTune_ >= tuning_.retune_interval);
+  }
+  uint32_t tune(float dt, uint32_t slot0) {
+    ensureEvents(slot0 + kTuneSteps);
+    for (uint32_t r = 0; r < kTuneReps; ++r) {
+      for (int v = 0; v < 3; ++v) {
+        for (int k = 0; k < 3; ++k) tuning_.tile_cap[k] = kTileCaps[v];
+        enqueueStep(dt, (int)(slot0 + r * 3 + v));
+        swapAfterStep();
+      }
+    }
+    NBS_HIP(hipEventSynchronize(events_[5 * (slot0 + kTuneSteps - 1) + 4]));
+    float best[3] = {3.0e38f, 3.0e38f, 3.0e38f};
+    int choice[3] = {kTileCaps[2], kTileCaps[2], kTileCaps[2]};
+    const int evPair[3][2] = {{0, 1}, {2, 3}, {3, 4}};  // viscosity, density, relaxation
+    for (int v = 0; v < 3; ++v) {
+      float t[3] = {3.0e38f, 3.0e38f, 3.0e38f};
+      for (uint32_t r = 0; r < kTuneReps; ++r) {
+        const uint32_t sl = slot0 + r * 3 + v;
+        for (int k = 0; k < 3; ++k) {
+          float ms = 0.0f;
+          if (hipEventElapsedTime(&ms, events_[5 * sl + evPair[k][0]], events_[5 * sl + evPair[k][1]]) == hipSuccess)
+            t[k] = std::min(t[k], ms);
+        }
+      }
+      for (int k = 0; k < 3; ++k)
+        if (t[k] < best[k]) best[k] = t[k], choice[k] = kTileCaps[v];
+    }
+    for (int k = 0; k < 3; ++k) tuning_.tile_cap[k] = choice[k];
+    tuned_ = true;
+    stepsSinceTune_ = 0;
+    ++tuneCount_;
+    destroyGraph();
+    return kTuneSteps;
+  }
+
   void swapAfterStep() {
     std::swap(buf_.id, buf_.id2);
     d_.id.swap(d_.id2);
@@ -587,17 +669,17 @@ class Simulation {
     lists_ = true;
   }
 
-  void ensureEvents(uint32_t nsteps) {
-    const size_t want = 5 * (size_t)std::max<uint32_t>(nsteps, 1);
+  void ensureEvents(uint32_t slots) {
+    const size_t want = 5 * (size_t)std::max<uint32_t>(slots, 1);
     while (events_.size() < want) {
       hipEvent_t e;
       NBS_HIP(hipEventCreate(&e));
       events_.push_back(e);
     }
   }
-  void collectTimes(uint32_t nsteps) {
+  void collectTimes(uint32_t slots) {
     float last[4] = {0, 0, 0, 0};
-    for (uint32_t s = 0; s < nsteps; ++s) {
+    for (uint32_t s = 0; s < slots; ++s) {
       for (int k = 0; k < 4; ++k) {
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, events_[5 * s + k], events_[5 * s + k + 1]) != hipSuccess) continue;
@@ -620,7 +702,7 @@ class Simulation {
   // Captures two steps (both buffer parities) as one graph.
   void ensureGraph(float dt) {
     const DevParams p = devParams(dt);
-    if (graphExec_ && std::memcmp(&p, &graphParams_, sizeof(p)) == 0 && graphN_ == n_) return;
+    if (graphExec_ && std::memcmp(&p, &graphParams_, sizeof(p)) == 0 && graphN_ == n_) return;  // caps: destroyGraph on change
     destroyGraph();
     const Buffers saved = buf_;
     NBS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
@@ -690,6 +772,7 @@ class Simulation {
   struct Storage {
     DevBuf<float2> x0, xp, x1, prev, v, dens, pres, acc;
     DevBuf<uint32_t> id, id2, keys, keyp, fill, tmpSlot, tmpId, scn, count, startCur, startNext, blockSum, blockOff;
+    DevBuf<uint32_t> candStats, cellStats;
   };
 
   nbs_params params_;
@@ -709,7 +792,6 @@ class Simulation {
   DevBuf<uint32_t> devSceneBase_;
   DevBuf<DevBody> devBodies_;
   DevBuf<float2> devVerts_;
-  uint32_t *stats_ = nullptr;
   DevBuf<float2> stX_, stPrev_, stV_, stDens_, stPres_;
   DevBuf<uint32_t> stKey_;
   DevBuf<int32_t> stScene_;
@@ -725,6 +807,11 @@ class Simulation {
   std::vector<hipEvent_t> events_;
   double kernelMs_[NBS_KERNEL_COUNT] = {};
   uint64_t kernelLaunches_[NBS_KERNEL_COUNT] = {};
+
+  nbs_tuning tuning_{1, 256, {2048, 2048, 2048}};
+  bool tuned_ = false;
+  uint64_t stepsSinceTune_ = 0;
+  uint64_t tuneCount_ = 0;
 
   hipGraphExec_t graphExec_ = nullptr;
   DevParams graphParams_{};
@@ -953,6 +1040,19 @@ nbs_status nbs_set_timing(nbs_sim *sim, int32_t enabled) {
 nbs_status nbs_set_options(nbs_sim *sim, uint32_t flags) {
   NBS_CHECK_SIM(sim);
   return guarded([&] { sim->impl->setOptions(flags); });
+}
+
+nbs_status nbs_set_tuning(nbs_sim *sim, const nbs_tuning *tuning) {
+  NBS_CHECK_SIM(sim);
+  if (!tuning) return NBS_ERR_INVALID_ARGUMENT;
+  return guarded([&] { sim->impl->setTuning(*tuning); });
+}
+
+nbs_status nbs_get_tuning(nbs_sim *sim, nbs_tuning *tuning) {
+  NBS_CHECK_SIM(sim);
+  if (!tuning) return NBS_ERR_INVALID_ARGUMENT;
+  *tuning = sim->impl->tuning();
+  return NBS_OK;
 }
 
 nbs_status nbs_get_diagnostics(nbs_sim *sim, nbs_diagnostics *out) {

@@ -18,7 +18,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import NbsDiagnostics, NbsDomain, NbsParams, NbsStats, call
+from ._lib import NbsDiagnostics, NbsDomain, NbsParams, NbsStats, NbsTuning, call
 from .scenes import SPHParams
 
 __all__ = ["ParticleSimulation", "NbsError", "params_to_c"]
@@ -271,6 +271,16 @@ class ParticleSimulation:
     def set_options(self, hip_graph: bool = False, check_finite: bool = False):
         flags = (_lib.OPT_HIP_GRAPH if hip_graph else 0) | (_lib.OPT_CHECK_FINITE if check_finite else 0)
         call("nbs_set_options", self._h, flags)
+
+    def get_tuning(self) -> dict:
+        t = NbsTuning()
+        call("nbs_get_tuning", self._h, ctypes.byref(t))
+        return {"autotune": bool(t.autotune), "retune_interval": int(t.retune_interval),
+                "tile_cap": [int(c) for c in t.tile_cap]}
+
+    def set_tuning(self, autotune: bool = True, retune_interval: int = 256, tile_cap=(2048, 2048, 2048)):
+        t = NbsTuning(1 if autotune else 0, int(retune_interval), (ctypes.c_int32 * 3)(*tile_cap))
+        call("nbs_set_tuning", self._h, ctypes.byref(t))
 
     def kernel_times(self) -> dict:
         avg = np.zeros(len(_lib.KERNEL_NAMES), np.float64)

@@ -57,3 +57,4 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.NbsDomain) == 12
     assert ctypes.sizeof(_lib.NbsStats) == 4 * 8 + 9 * 4 + 4  # padded to 8
     assert ctypes.sizeof(_lib.NbsDiagnostics) == 8 + 16 + 8 * 5 + 16 + 8
+    assert ctypes.sizeof(_lib.NbsTuning) == 20

@@ -141,3 +141,27 @@ def test_dam_break_1m_three_steps_bit_exact():
     sim.Update(DT, 3)
     ref.step(DT, 3)
     assert_bit_exact(sim.get_state(), ref.state(), "1M step 3: ")
+
+
+@pytest.mark.parametrize("caps", [(0, 0, 0), (1024, 1024, 1024), (2048, 2048, 2048), (0, 2048, 1024)])
+def test_tile_capacities_bit_exact(caps):
+    spec = dam_break_4k()
+    sim = gpu_for([spec])
+    sim.set_tuning(autotune=False, tile_cap=caps)
+    ref = oracle_for([spec])
+    sim.Update(DT, 150)
+    ref.step(DT, 150)
+    assert_bit_exact(sim.get_state(), ref.state(), f"caps {caps}: ")
+
+
+def test_autotuned_batched_run_bit_exact():
+    specs = batched_scenes(seed=8, count=12, n_min=2048, n_max=8192)
+    sim = gpu_for(specs)
+    sim.set_tuning(autotune=True, retune_interval=16)
+    sim.set_options(hip_graph=True)
+    ref = oracle_for(specs)
+    sim.Update(DT, 60)
+    ref.step(DT, 60)
+    assert_bit_exact(sim.get_state(), ref.state(), "autotuned batched: ")
+    t = sim.get_tuning()
+    assert all(c in (0, 1024, 2048) for c in t["tile_cap"])
