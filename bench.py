@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
+                    help="also run the library's copy kernel on MiB of data (3 times) for counter calibration")
     ap.add_argument("--tile-cap", default="auto",
                     help="LDS tile capacities of the viscosity, density and relaxation kernels, e.g. 2048,2048,1024; "
                          "'auto' lets the library's autotuner choose")
@@ -96,6 +98,12 @@ def main():
 
     from nbodysimulation_experiment_amd.sim import ParticleSimulation
     from nbodysimulation_experiment_amd.scenes import SPHParams
+
+    from nbodysimulation_experiment_amd.sim import bench_copy
+    if args.calibrate_copy:
+        log(f"copy kernel on {args.calibrate_copy} MiB: {bench_copy(args.calibrate_copy << 20, 3):.1f} GB/s")
+    copy_gbs = bench_copy(1 << 30, 20)
+    log(f"device copy ceiling {copy_gbs:.1f} GB/s")
 
     specs, desc = build_workload(args.workload)
     n = sum(s.n for s in specs)
@@ -145,6 +153,7 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
                     "kernel_avg_ms": round(avg_ms, 5), "alg_bytes_per_launch": alg,
+                    "copy_ceiling_gbs": round(copy_gbs, 1),
                     "step_alg_bytes_frac": round(value * STEP_ALG_BYTES / 1e9 / HBM_PEAK_GBS, 5)}
 
     cpu = None

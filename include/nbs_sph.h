@@ -224,10 +224,18 @@ nbs_status nbs_get_grid(nbs_sim *sim, uint32_t *sorted_ids, uint32_t *cell_start
  * parallelism, so support is reported as 0 and the worker count as 1. */
 nbs_status nbs_is_multithreading_supported(nbs_sim *sim, int32_t *supported);
 
-/* Device arithmetic self-test: out[4*i..4*i+3] = sqrtf(a[i]), 1.0f/a[i],
- * a[i]/b[i], (float)cell coordinate of a[i] for (half = b[i], size = 1, n = 1<<20).
- * Used to show the device's FP32 sqrt and divide are correctly rounded. */
+/* Device arithmetic self-test: out[6*i .. 6*i+5] = sqrtf(a[i]), 1.0f/a[i],
+ * a[i]/b[i], the cell coordinate of a[i] for (half = b[i], size = 1,
+ * n = 1<<20), and the pair kernels' correctly rounded square root and
+ * reciprocal of a[i].  Used to show the device's FP32 arithmetic is the
+ * correctly rounded IEEE arithmetic the reference's CPU build performs. */
 nbs_status nbs_selftest_math(const float *a, const float *b, uint64_t n, float *out);
+
+/* Device copy-bandwidth probe: copies `bytes` between two device buffers
+ * `reps` times and returns read+write GB/s.  No reference counterpart: it is
+ * the measured HBM ceiling the step's roofline is compared with, and a kernel
+ * of known traffic for calibrating the FETCH_SIZE / WRITE_SIZE counters. */
+nbs_status nbs_bench_copy(uint64_t bytes, int32_t reps, double *gbps);
 
 /* Human-readable name of a status code. */
 const char *nbs_status_string(int32_t status);

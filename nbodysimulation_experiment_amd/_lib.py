@@ -107,6 +107,7 @@ SIGNATURES = {
     "nbs_get_grid": [vp, vp, vp],
     "nbs_is_multithreading_supported": [vp, P(c_int32)],
     "nbs_selftest_math": [vp, vp, c_uint64, vp],
+    "nbs_bench_copy": [c_uint64, c_int32, P(c_double)],
 }
 
 

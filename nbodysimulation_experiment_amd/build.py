@@ -50,5 +50,25 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+DRIVER = os.path.join(ROOT, "tests", "cpp", "demo5_driver")
+
+
+def build_host_driver(verbose: bool = False) -> str:
+    """Builds the C++ BaseSimulation wrapper (host/demo5.cpp) into the test
+    driver tests/cpp/demo5_driver, linked against libnbs_sph.so."""
+    host = os.path.join(HERE, "host")
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{os.path.join(ROOT, 'include')}",
+           f"-I{host}", os.path.join(host, "demo5.cpp"), DRIVER + ".cpp", f"-L{HERE}", "-lnbs_sph",
+           f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../nbodysimulation_experiment_amd", "-o", DRIVER]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"host driver build failed ({r.returncode})")
+    return DRIVER
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_host_driver(verbose=True))

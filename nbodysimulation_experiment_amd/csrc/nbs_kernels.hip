@@ -4,7 +4,8 @@
 //   1. k_visc_predict   integrate, viscosity (Jacobi gather over the previous
 //                       binning), predict, cell key of the predicted position and
 //                       the per-cell arrival count (demo4.cpp:301-356)
-//   2. k_scan_*         exclusive scan of the per-cell counts -> cell starts
+//   2. k_scan_reduce + k_scan_final
+//                       exclusive scan of the per-cell counts -> cell starts
 //   3. k_scatter + k_rank_reorder
 //                       bucket the particles by cell, order each cell by particle
 //                       index, and move the state into the new slot order
@@ -16,6 +17,8 @@
 // Every FP32 expression keeps the reference's operand order and the library
 // is compiled with -ffp-contract=off and correctly rounded sqrt / divide, so the
 // arithmetic of each pair term is the reference's (sph.h:465-542).
+#include <algorithm>
+
 #include "nbs_kernels.hpp"
 
 namespace nbs {
@@ -55,6 +58,36 @@ template <bool MULTI>
 __device__ __forceinline__ DevScene scene_of_slot(const Buffers &b, const DevScene &s0, uint32_t key) {
   if (!MULTI) return s0;
   return b.scenes[scene_of_key(b.sceneBase, b.nScenes, key)];
+}
+
+// Correctly rounded FP32 reciprocal and square root in fewer instructions than
+// the general IEEE lowering, for the operand ranges of the pair kernels.
+//
+// rcp_rn: v_rcp_f32 (within 1 ulp) refined by one Newton step with FMA
+// (Markstein): e = 1 - x*y exactly, y' = y + e*y.  Used for x in [2^-60, 2^60];
+// other operands take the IEEE division.
+// sqrt_rn: v_sqrt_f32 followed by the residual test of the neighbouring floats
+// (s - ulp, s + ulp): x - s'*s computed with one FMA decides the rounding.  Used
+// for x == 0 and x in [2^-96, 2^100]; other operands take sqrtf.
+// Both are checked against the correctly rounded results for every mantissa of
+// several binades by the device arithmetic self-test.
+__device__ __forceinline__ float rcp_rn(float x) {
+  if (!(x >= 0x1p-60f && x <= 0x1p60f)) return 1.0f / x;
+  const float y = __builtin_amdgcn_rcpf(x);
+  const float e = __builtin_fmaf(-x, y, 1.0f);
+  return __builtin_fmaf(e, y, y);
+}
+
+__device__ __forceinline__ float sqrt_rn(float x) {
+  if (!((x >= 0x1p-96f && x <= 0x1p100f) || x == 0.0f)) return sqrtf(x);
+  float s = __builtin_amdgcn_sqrtf(x);
+  const float dn = __uint_as_float(__float_as_uint(s) - 1u);
+  const float up = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rdn = __builtin_fmaf(-dn, s, x);
+  const float rup = __builtin_fmaf(-up, s, x);
+  if (rdn <= 0.0f) s = dn;
+  if (rup > 0.0f) s = up;
+  return s;
 }
 
 // ---------------------------------------------------------------------------
@@ -300,6 +333,34 @@ __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const D
   return cand;
 }
 
+// Arrival index of this lane's particle in cell `key`: lanes of the wave with
+// equal keys are grouped, the lowest lane of each group adds the group size to
+// the cell counter with one atomic, and each lane takes base + its rank in the
+// group.  Consecutive sorted slots mostly share a cell, so this issues a few
+// atomics per wave instead of one per particle.  All active lanes must call it.
+__device__ __forceinline__ uint32_t wave_aggregated_fill(uint32_t *count, uint32_t key) {
+  const int lane = (int)(threadIdx.x & 63u);
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint64_t todo = __ballot(1);
+  int leader = lane;
+  uint32_t rank = 0, size = 0;
+  while (todo) {
+    const int l = __builtin_ctzll(todo);
+    const uint32_t k = (uint32_t)__shfl((int)key, l, 64);
+    const uint64_t m = __ballot(key == k) & todo;
+    if ((m >> lane) & 1ull) {
+      leader = l;
+      rank = (uint32_t)__popcll(m & below);
+      size = (uint32_t)__popcll(m);
+    }
+    todo &= ~m;
+  }
+  uint32_t base = 0;
+  if (leader == lane) base = atomicAdd(&count[key], size);
+  base = (uint32_t)__shfl((int)base, leader, 64);
+  return base + rank;
+}
+
 // ---------------------------------------------------------------------------
 // 1. integrate + viscosity + predict + key + count
 // ---------------------------------------------------------------------------
@@ -312,10 +373,9 @@ __device__ __forceinline__ void viscosity_sum(const uint32_t *start, const DevSc
   for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t q, float Rx, float Ry, float r2) {
     const float2 vj = V[q];
     const float vjx = p.gdtx + vj.x, vjy = p.gdty + vj.y;
-    const float r = sqrtf(r2);                            // sph.h:503
+    const float r = sqrt_rn(r2);                          // sph.h:503
     const float qq = r * p.invH;                          // sph.h:504
-    const float l = (r == 0.0f) ? 1.0f : r;               // sph.h:505 Vec2Normalize
-    const float inv = 1.0f / l;
+    const float inv = (r == 0.0f) ? 1.0f : rcp_rn(r);     // sph.h:505 Vec2Normalize (length 0 -> 1)
     const float nX = Rx * inv, nY = Ry * inv;
     const float u = (vx - vjx) * nX + (vy - vjy) * nY;    // sph.h:506
     if (u > 0.0f) {
@@ -392,7 +452,7 @@ __global__ __launch_bounds__(kBlock) void k_visc_predict(Buffers b, DevParams p,
   const int cy = cell_coord(xpy, sc.halfH, sc.cellSize, sc.ny);
   const uint32_t key = sc.cellBase + (uint32_t)(cy * sc.nx + cx);
   b.keyp[i] = key;
-  b.fill[i] = atomicAdd(&b.count[key], 1u);
+  b.fill[i] = wave_aggregated_fill(b.count, key);
 }
 
 // Restores the per-slot scene and zero acceleration of particles that lose
@@ -482,19 +542,6 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(Buffers b, uint32_t cell
   if (threadIdx.x == 0) b.blockSum[blockIdx.x] = tot;
 }
 
-// One workgroup: exclusive scan of the workgroup sums.
-__global__ __launch_bounds__(kBlock) void k_scan_blocks(Buffers b, uint32_t nblocks) {
-  uint32_t carry = 0;
-  for (uint32_t base = 0; base < nblocks; base += kBlock) {
-    const uint32_t k = base + threadIdx.x;
-    const uint32_t v = (k < nblocks) ? b.blockSum[k] : 0u;
-    uint32_t tot;
-    const uint32_t ex = block_excl_scan(v, &tot);
-    if (k < nblocks) b.blockOff[k] = carry + ex;
-    carry += tot;
-  }
-}
-
 __global__ __launch_bounds__(kBlock) void k_scan_final(Buffers b, uint32_t cells) {
   const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
@@ -505,8 +552,18 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(Buffers b, uint32_t cells
     v[k] = (c < cells) ? b.count[c] : 0u;
     s += v[k];
   }
+  // offset of this workgroup: the sum of the preceding workgroups' totals
+  __shared__ uint32_t shOff[kBlock / 64];
+  uint32_t pre = 0;
+  for (uint32_t q = threadIdx.x; q < blockIdx.x; q += kBlock) pre += b.blockSum[q];
+  for (int d = 32; d >= 1; d >>= 1) pre += (uint32_t)__shfl_xor((int)pre, d, 64);
+  if ((threadIdx.x & 63u) == 0) shOff[threadIdx.x >> 6] = pre;
+  __syncthreads();
+  uint32_t blockOff = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) blockOff += shOff[w];
   uint32_t tot;
-  uint32_t run = b.blockOff[blockIdx.x] + block_excl_scan(s, &tot);
+  uint32_t run = blockOff + block_excl_scan(s, &tot);
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     const uint64_t c = base + k;
@@ -555,7 +612,7 @@ __device__ __forceinline__ uint32_t density_sum(const uint32_t *start, const Dev
                                                 const float2 *X, Idx idx, const DevParams &p, float2 xi, float &d0,
                                                 float &d1) {
   return for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t, float, float, float r2) {
-    const float r = sqrtf(r2);                            // sph.h:471
+    const float r = sqrt_rn(r2);                          // sph.h:471
     const float term = 1.0f - r * p.invH;                 // sph.h:472
     d0 = d0 + (term * term);                              // sph.h:473
     d1 = d1 + ((term * term) * term);                     // sph.h:474
@@ -620,9 +677,8 @@ __device__ __forceinline__ void delta_sum(const uint32_t *start, const DevScene 
                                           float &accx, float &accy) {
   for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t q, float Rx, float Ry, float r2) {
     const float2 pj = P[q];
-    const float r = sqrtf(r2);                            // sph.h:489
-    const float l = (r == 0.0f) ? 1.0f : r;               // sph.h:490 Vec2Normalize
-    const float inv = 1.0f / l;
+    const float r = sqrt_rn(r2);                          // sph.h:489
+    const float inv = (r == 0.0f) ? 1.0f : rcp_rn(r);     // sph.h:490 Vec2Normalize (length 0 -> 1)
     const float nX = Rx * inv, nY = Ry * inv;
     const float term = 1.0f - r * p.invH;                 // sph.h:491
     const float tt = term * term;
@@ -781,10 +837,17 @@ __global__ __launch_bounds__(kBlock) void k_selftest_math(const float *a, const 
   const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const float x = a[i], y = b[i];
-  out[4 * i + 0] = sqrtf(x);
-  out[4 * i + 1] = 1.0f / x;
-  out[4 * i + 2] = x / y;
-  out[4 * i + 3] = (float)cell_coord(x, y, 1.0f, 1 << 20);
+  out[6 * i + 0] = sqrtf(x);
+  out[6 * i + 1] = 1.0f / x;
+  out[6 * i + 2] = x / y;
+  out[6 * i + 3] = (float)cell_coord(x, y, 1.0f, 1 << 20);
+  out[6 * i + 4] = sqrt_rn(x);
+  out[6 * i + 5] = rcp_rn(x);
+}
+
+// Plain device copy (the bandwidth probe and counter calibration).
+__global__ __launch_bounds__(kBlock) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -820,7 +883,6 @@ void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s
 void launch_scan(const Buffers &b, uint32_t cells, hipStream_t s) {
   const uint32_t nb = (cells + kScanChunk - 1) / kScanChunk;
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, b, cells);
-  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kBlock), 0, s, b, nb);
   hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kBlock), 0, s, b, cells);
 }
 
@@ -880,6 +942,13 @@ void launch_diagnostics(const Buffers &b, const DevParams &p, int n, double *par
   const int nb = diagnostics_blocks(n);
   if (nb > 0) hipLaunchKernelGGL(k_diag_partial, dim3(nb), dim3(kBlock), 0, s, b, p, n, partials);
   hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kBlock), 0, s, nb, partials, out);
+}
+
+void launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s) {
+  const uint64_t n = bytes / 16;
+  if (n == 0) return;
+  const unsigned blocks = (unsigned)std::min<uint64_t>((n + kBlock - 1) / kBlock, 256ull * 64);
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(kBlock), 0, s, (const float4 *)src, (float4 *)dst, n);
 }
 
 void launch_selftest_math(const float *a, const float *b, uint64_t n, float *out, hipStream_t s) {

@@ -62,7 +62,7 @@ struct Buffers {
   uint32_t *count;     // per cell, zero between steps
   uint32_t *startCur;  // per cell + 1: the last binning
   uint32_t *startNext; // per cell + 1: the binning being built
-  uint32_t *blockSum, *blockOff;
+  uint32_t *blockSum;  // per scan workgroup: its cells' total
   uint32_t *candStats; // per density workgroup: min, max candidate count
   uint32_t *cellStats; // per scan workgroup: min, max occupied-cell count
   const DevScene *scenes;
@@ -89,6 +89,7 @@ void launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t stream
 void launch_diagnostics(const Buffers &b, const DevParams &p, int n, double *partials, double *out,
                         hipStream_t stream);
 int diagnostics_blocks(int n);
+void launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t stream);
 void launch_selftest_math(const float *a, const float *b, uint64_t n, float *out, hipStream_t stream);
 
 }  // namespace nbs

@@ -519,7 +519,6 @@ class Simulation {
     d_.startCur.alloc(c + 1);
     d_.startNext.alloc(c + 1);
     d_.blockSum.alloc(std::max<size_t>(nb, 1));
-    d_.blockOff.alloc(std::max<size_t>(nb, 1));
     d_.cellStats.alloc(2 * std::max<size_t>(nb, 1));
     launch_fill_u32(d_.count.p, 0u, c, stream_);
     launch_fill_u32(d_.startCur.p, 0u, c + 1, stream_);
@@ -552,7 +551,7 @@ class Simulation {
     buf_.id = d_.id.p; buf_.id2 = d_.id2.p; buf_.keys = d_.keys.p; buf_.keyp = d_.keyp.p; buf_.fill = d_.fill.p;
     buf_.tmpSlot = d_.tmpSlot.p; buf_.tmpId = d_.tmpId.p; buf_.scn = d_.scn.p;
     buf_.count = d_.count.p; buf_.startCur = d_.startCur.p; buf_.startNext = d_.startNext.p;
-    buf_.blockSum = d_.blockSum.p; buf_.blockOff = d_.blockOff.p;
+    buf_.blockSum = d_.blockSum.p;
     buf_.candStats = d_.candStats.p; buf_.cellStats = d_.cellStats.p;
     buf_.scenes = devScenes_.p; buf_.sceneBase = devSceneBase_.p; buf_.nScenes = (int)scenes_.size();
     buf_.bodies = devBodies_.p; buf_.verts = devVerts_.p;
@@ -771,7 +770,7 @@ class Simulation {
 
   struct Storage {
     DevBuf<float2> x0, xp, x1, prev, v, dens, pres, acc;
-    DevBuf<uint32_t> id, id2, keys, keyp, fill, tmpSlot, tmpId, scn, count, startCur, startNext, blockSum, blockOff;
+    DevBuf<uint32_t> id, id2, keys, keyp, fill, tmpSlot, tmpId, scn, count, startCur, startNext, blockSum;
     DevBuf<uint32_t> candStats, cellStats;
   };
 
@@ -1088,12 +1087,36 @@ nbs_status nbs_selftest_math(const float *a, const float *b, uint64_t n, float *
     nbs::DevBuf<float> da, db, dout;
     da.alloc(n);
     db.alloc(n);
-    dout.alloc(4 * n);
+    dout.alloc(6 * n);
     NBS_HIP(hipMemcpy(da.p, a, n * 4, hipMemcpyHostToDevice));
     NBS_HIP(hipMemcpy(db.p, b, n * 4, hipMemcpyHostToDevice));
     nbs::launch_selftest_math(da.p, db.p, n, dout.p, nullptr);
     NBS_HIP(hipGetLastError());
-    NBS_HIP(hipMemcpy(out, dout.p, 16 * n, hipMemcpyDeviceToHost));
+    NBS_HIP(hipMemcpy(out, dout.p, 24 * n, hipMemcpyDeviceToHost));
+  });
+}
+
+nbs_status nbs_bench_copy(uint64_t bytes, int32_t reps, double *gbps) {
+  if (!gbps || reps <= 0 || bytes < 16) return NBS_ERR_INVALID_ARGUMENT;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return NBS_ERR_NO_DEVICE;
+  return guarded([&] {
+    nbs::DevBuf<uint8_t> a, b;
+    a.alloc(bytes);
+    b.alloc(bytes);
+    hipEvent_t e0, e1;
+    NBS_HIP(hipEventCreate(&e0));
+    NBS_HIP(hipEventCreate(&e1));
+    nbs::launch_copy(a.p, b.p, bytes, nullptr);  // first touch
+    NBS_HIP(hipEventRecord(e0, nullptr));
+    for (int32_t r = 0; r < reps; ++r) nbs::launch_copy(a.p, b.p, bytes, nullptr);
+    NBS_HIP(hipEventRecord(e1, nullptr));
+    NBS_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    NBS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *gbps = 2.0 * (double)(bytes / 16 * 16) * reps / ((double)ms * 1e-3) / 1e9;
   });
 }
 

@@ -292,6 +292,13 @@ class ParticleSimulation:
 def selftest_math(a, b) -> np.ndarray:
     a = np.ascontiguousarray(a, np.float32)
     b = np.ascontiguousarray(b, np.float32)
-    out = np.zeros((a.size, 4), np.float32)
+    out = np.zeros((a.size, 6), np.float32)
     call("nbs_selftest_math", _ptr(a), _ptr(b), a.size, _ptr(out))
     return out
+
+
+def bench_copy(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """Device copy bandwidth (read + write GB/s) of the library's copy kernel."""
+    g = ctypes.c_double()
+    call("nbs_bench_copy", int(nbytes), int(reps), ctypes.byref(g))
+    return float(g.value)

@@ -30,10 +30,45 @@ def test_device_sqrt_and_divide_are_correctly_rounded():
                         [0.0, 1.0, 2.0, 4.0, 0.25, 1e-38, 1e-45, 3.4e38]]).astype(np.float32)
     b = rng.uniform(0.1, 100, a.size).astype(np.float32)
     out = selftest_math(a, b)
-    with np.errstate(divide="ignore"):
+    with np.errstate(divide="ignore", over="ignore"):
         np.testing.assert_array_equal(out[:, 0].view(np.uint32), np.sqrt(a).view(np.uint32))
         np.testing.assert_array_equal(out[:, 1].view(np.uint32), (np.float32(1) / a).view(np.uint32))
         np.testing.assert_array_equal(out[:, 2].view(np.uint32), (a / b).view(np.uint32))
+        np.testing.assert_array_equal(out[:, 4].view(np.uint32), np.sqrt(a).view(np.uint32))
+        np.testing.assert_array_equal(out[:, 5].view(np.uint32), (np.float32(1) / a).view(np.uint32))
+
+
+def _binade(e):
+    mant = np.arange(1 << 23, dtype=np.uint32)
+    return ((np.uint32(e + 127) << np.uint32(23)) | mant).view(np.float32)
+
+
+def test_pair_kernel_rcp_and_sqrt_exhaustive_binades():
+    """Every mantissa of the binades spanning the fast paths' ranges: the pair
+    kernels' reciprocal and square root equal the correctly rounded results."""
+    from nbodysimulation_experiment_amd.sim import selftest_math
+    one = None
+    for e in (-60, -47, -20, -1, 0, 1, 7, 33, 59, 60):
+        a = _binade(e)
+        one = np.ones_like(a) if one is None else one
+        out = selftest_math(a, one)
+        with np.errstate(over="ignore"):
+            np.testing.assert_array_equal(out[:, 5].view(np.uint32), (np.float32(1) / a).view(np.uint32),
+                                          err_msg=f"rcp binade 2^{e}")
+            np.testing.assert_array_equal(out[:, 1].view(np.uint32), (np.float32(1) / a).view(np.uint32))
+    for e in (-96, -95, -41, -2, -1, 0, 1, 40, 99, 100):
+        a = _binade(e)
+        out = selftest_math(a, one)
+        np.testing.assert_array_equal(out[:, 4].view(np.uint32), np.sqrt(a).view(np.uint32),
+                                      err_msg=f"sqrt binade 2^{e}")
+        np.testing.assert_array_equal(out[:, 0].view(np.uint32), np.sqrt(a).view(np.uint32))
+    # fallback ranges and special values
+    sp = np.array([0.0, -0.0, 1e-45, 1e-40, 2.0 ** -97, 2.0 ** -61, 2.0 ** 61, 2.0 ** 101, 3.4e38, np.inf],
+                  np.float32)
+    out = selftest_math(sp, np.ones_like(sp))
+    with np.errstate(divide="ignore", over="ignore"):
+        np.testing.assert_array_equal(out[:, 4].view(np.uint32), np.sqrt(sp).view(np.uint32))
+        np.testing.assert_array_equal(out[:, 5].view(np.uint32), (np.float32(1) / sp).view(np.uint32))
 
 
 def test_device_cell_coordinate_matches_oracle():
@@ -43,7 +78,7 @@ def test_device_cell_coordinate_matches_oracle():
     p = np.concatenate([rng.uniform(-50, 50, 20000), np.arange(-45, 45, 0.25), [np.nan, np.inf, -np.inf, 3e38, -3e38]])
     p = p.astype(np.float32)
     half = np.full(p.size, 40.0, np.float32)
-    out = selftest_math(p, half)
+    out = selftest_math(p, half)[:, :4]
     # kat_cell with an 80 x 1<<20-wide grid: the x coordinate of (p, 0)
     ref = kat_cell(80.0, 80.0, 1.0, np.stack([p, np.zeros_like(p)], 1))
     want = np.minimum(np.maximum(ref[:, 0], 0), 1 << 20)
