@@ -46,11 +46,11 @@ def log(msg):
 
 def build_workload(name):
     from nbodysimulation_experiment_amd import scenes
-    if name == "cfg1":
+    if name == "dam-break-1m":
         return [scenes.dam_break_1m()], "dam break, 1,048,576 particles (1024x1024 at 0.5h), 600h x 600h box, 4 walls + 64 circles"
-    if name == "cfg0":
+    if name == "dam-break-4k":
         return [scenes.dam_break_4k()], "dam break, 4,096 particles (64x64 at 0.5h), 80h x 60h box, 4 walls"
-    if name == "cfg2":
+    if name == "batched-256":
         sp = scenes.batched_scenes()
         return sp, f"{len(sp)} independent scenes, {sum(s.n for s in sp):,} particles, counts log-uniform in [2048, 65536]"
     raise SystemExit(f"unknown workload {name}")
@@ -80,7 +80,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2"])
+    ap.add_argument("--workload", default="dam-break-1m", choices=["dam-break-4k", "dam-break-1m", "batched-256"])
     ap.add_argument("--graph", action="store_true", help="replay steps as HIP graphs (no per-kernel timing)")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-threads", type=int, default=0)

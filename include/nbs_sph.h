@@ -197,16 +197,16 @@ nbs_status nbs_set_timing(nbs_sim *sim, int32_t enabled);
 /* NBS_OPT_* flags. */
 nbs_status nbs_set_options(nbs_sim *sim, uint32_t flags);
 
-/* Kernel-shape tuning.  The pair kernels stage each workgroup's candidate
- * rows in LDS tiles of tile_cap particles (0: no staging, read neighbours from
- * HBM/L2).  With autotune on, the library times every capacity on real steps
+/* Kernel-shape tuning.  The pair kernels stage the positions of each
+ * workgroup's candidate rows in LDS tiles of tile_cap particles (0: no staging,
+ * read neighbours from HBM/L2).  With autotune on, the library times every capacity on real steps
  * (all capacities give bit-identical results) and keeps the fastest per
- * kernel, again every retune_interval steps (0: once).  No reference
+ * kernel, again every retune_interval steps (0: once; default 64).  No reference
  * counterpart: the reference's only tunable is its thread count. */
 typedef struct nbs_tuning {
   int32_t autotune;
   uint32_t retune_interval;
-  int32_t tile_cap[3]; /* viscosity, density, relaxation: 0, 1024 or 2048 */
+  int32_t tile_cap[3]; /* viscosity, density, relaxation: 0, 1024, 2048 or 4096 */
 } nbs_tuning;
 nbs_status nbs_set_tuning(nbs_sim *sim, const nbs_tuning *tuning);
 nbs_status nbs_get_tuning(nbs_sim *sim, nbs_tuning *tuning);

@@ -242,7 +242,9 @@ __device__ __forceinline__ RowRange row_range(const uint32_t *start, const DevSc
   return {start[row + xlo], start[row + xhi + 1]};
 }
 
-// LDS tile of a workgroup.  The workgroup's particles occupy consecutive
+// LDS tile of a workgroup (positions only: every candidate's position is read
+// by the cheap pass, the other per-particle data only for in-kernel pairs, from
+// the slot arrays).  The workgroup's particles occupy consecutive
 // sorted slots, so their keys span [kA, kB] and the candidates of all of them
 // lie in three slot ranges, one per row offset dy: the cells with keys in
 // [kA + dy*nx - 1, kB + dy*nx + 1] (clamped to the scene).  Each range is
@@ -294,9 +296,9 @@ struct SlotIndex {
   __device__ __forceinline__ uint32_t operator()(int, uint32_t j) const { return j; }
 };
 
-// Calls f(q, Rx, Ry, r2) for every candidate of the particle at xi in cell
-// (cx, cy) whose r2 < h2 (sph.h:470, 488, 502), in candidate order, where q
-// indexes the candidate in the data source (tile or slot arrays); returns the
+// Calls f(j, Rx, Ry, r2) for every candidate slot j of the particle at xi in
+// cell (cx, cy) whose r2 < h2 (sph.h:470, 488, 502), in candidate order, reading
+// positions from X at idx(d, j) (the LDS tile or the slot array); returns the
 // number of candidates (the reference's neighbour count).  Each row range is
 // taken in chunks of 64: a cheap pass marks the in-kernel candidates in a
 // 64-bit mask, then the pair work runs only for the marked ones, lowest first,
@@ -323,10 +325,9 @@ __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const D
       while (mask) {
         const uint32_t t = (uint32_t)__builtin_ctzll(mask);
         mask &= mask - 1u;
-        const uint32_t q = q0 + t;
-        const float2 xj = X[q];
+        const float2 xj = X[q0 + t];
         const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;
-        f(q, Rx, Ry, Rx * Rx + Ry * Ry);
+        f(base + t, Rx, Ry, Rx * Rx + Ry * Ry);
       }
     }
   }
@@ -370,8 +371,8 @@ template <typename Idx>
 __device__ __forceinline__ void viscosity_sum(const uint32_t *start, const DevScene &sc, int cx, int cy,
                                               const float2 *X, const float2 *V, Idx idx, const DevParams &p,
                                               float2 xi, float vx, float vy, float &accx, float &accy) {
-  for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t q, float Rx, float Ry, float r2) {
-    const float2 vj = V[q];
+  for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t j, float Rx, float Ry, float r2) {
+    const float2 vj = V[j];
     const float vjx = p.gdtx + vj.x, vjy = p.gdty + vj.y;
     const float r = sqrt_rn(r2);                          // sph.h:503
     const float qq = r * p.invH;                          // sph.h:504
@@ -393,7 +394,7 @@ __device__ __forceinline__ void viscosity_sum(const uint32_t *start, const DevSc
 template <bool MULTI, int CAP>
 __global__ __launch_bounds__(kBlock) void k_visc_predict(Buffers b, DevParams p, DevScene s0, int n, int nBinned,
                                                          int lists) {
-  __shared__ float2 tx[CAP > 0 ? CAP : 1], tv[CAP > 0 ? CAP : 1];
+  __shared__ float2 tx[CAP > 0 ? CAP : 1];
   __shared__ TilePlan plan;
   const int i0 = (int)(xcd_block() * kBlock);
   const int i = i0 + (int)threadIdx.x;
@@ -409,7 +410,6 @@ __global__ __launch_bounds__(kBlock) void k_visc_predict(Buffers b, DevParams p,
     __syncthreads();
     if (CAP > 0 && plan.use) {
       stage(plan, b.x0, tx);
-      stage(plan, b.v, tv);
     }
     __syncthreads();
   }
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void k_visc_predict(Buffers b, DevParams p,
     const int cx = (int)local - cy * sc.nx;
     float accx = 0.0f, accy = 0.0f;
     if (CAP > 0 && plan.use)
-      viscosity_sum(b.startCur, sc, cx, cy, tx, tv, TileIndex(plan), p, xi, vx, vy, accx, accy);
+      viscosity_sum(b.startCur, sc, cx, cy, tx, b.v, TileIndex(plan), p, xi, vx, vy, accx, accy);
     else
       viscosity_sum(b.startCur, sc, cx, cy, b.x0, b.v, SlotIndex(), p, xi, vx, vy, accx, accy);
     vx = accx + vx;
@@ -675,8 +675,8 @@ template <typename Idx>
 __device__ __forceinline__ void delta_sum(const uint32_t *start, const DevScene &sc, int cx, int cy, const float2 *X,
                                           const float2 *P, Idx idx, const DevParams &p, float2 xi, float2 pi,
                                           float &accx, float &accy) {
-  for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t q, float Rx, float Ry, float r2) {
-    const float2 pj = P[q];
+  for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t j, float Rx, float Ry, float r2) {
+    const float2 pj = P[j];
     const float r = sqrt_rn(r2);                          // sph.h:489
     const float inv = (r == 0.0f) ? 1.0f : rcp_rn(r);     // sph.h:490 Vec2Normalize (length 0 -> 1)
     const float nX = Rx * inv, nY = Ry * inv;
@@ -693,14 +693,13 @@ __device__ __forceinline__ void delta_sum(const uint32_t *start, const DevScene 
 
 template <bool MULTI, int CAP>
 __global__ __launch_bounds__(kBlock) void k_delta(Buffers b, DevParams p, DevScene s0, int n) {
-  __shared__ float2 tx[CAP > 0 ? CAP : 1], tp[CAP > 0 ? CAP : 1];
+  __shared__ float2 tx[CAP > 0 ? CAP : 1];
   __shared__ TilePlan plan;
   const int i0 = (int)(xcd_block() * kBlock);
   const int i = i0 + (int)threadIdx.x;
   plan_new<MULTI, CAP>(b, s0, i0, n, plan);
   if (CAP > 0 && plan.use) {
     stage(plan, b.x1, tx);
-    stage(plan, b.pres, tp);
   }
   __syncthreads();
   if (i >= n) return;
@@ -713,7 +712,7 @@ __global__ __launch_bounds__(kBlock) void k_delta(Buffers b, DevParams p, DevSce
   const float2 pi = b.pres[i];
   float accx = 0.0f, accy = 0.0f;
   if (CAP > 0 && plan.use)
-    delta_sum(b.startNext, sc, cx, cy, tx, tp, TileIndex(plan), p, xi, pi, accx, accy);
+    delta_sum(b.startNext, sc, cx, cy, tx, b.pres, TileIndex(plan), p, xi, pi, accx, accy);
   else
     delta_sum(b.startNext, sc, cx, cy, b.x1, b.pres, SlotIndex(), p, xi, pi, accx, accy);
   float x = accx + xi.x;                                  // demo4.cpp:253
@@ -876,7 +875,8 @@ void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s
   switch (cap) {
     case 0: visc_cap<0>(b, p, s0, n, nBinned, lists, s); break;
     case 1024: visc_cap<1024>(b, p, s0, n, nBinned, lists, s); break;
-    default: visc_cap<2048>(b, p, s0, n, nBinned, lists, s); break;
+    case 2048: visc_cap<2048>(b, p, s0, n, nBinned, lists, s); break;
+    default: visc_cap<4096>(b, p, s0, n, nBinned, lists, s); break;
   }
 }
 
@@ -905,7 +905,8 @@ void launch_density(const Buffers &b, const DevParams &p, const DevScene &s0, in
   switch (cap) {
     case 0: density_cap<0>(b, p, s0, n, s); break;
     case 1024: density_cap<1024>(b, p, s0, n, s); break;
-    default: density_cap<2048>(b, p, s0, n, s); break;
+    case 2048: density_cap<2048>(b, p, s0, n, s); break;
+    default: density_cap<4096>(b, p, s0, n, s); break;
   }
 }
 
@@ -922,7 +923,8 @@ void launch_delta(const Buffers &b, const DevParams &p, const DevScene &s0, int 
   switch (cap) {
     case 0: delta_cap<0>(b, p, s0, n, s); break;
     case 1024: delta_cap<1024>(b, p, s0, n, s); break;
-    default: delta_cap<2048>(b, p, s0, n, s); break;
+    case 2048: delta_cap<2048>(b, p, s0, n, s); break;
+    default: delta_cap<4096>(b, p, s0, n, s); break;
   }
 }
 

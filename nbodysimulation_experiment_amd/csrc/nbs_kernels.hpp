@@ -74,15 +74,17 @@ struct Buffers {
 
 // Launchers (nbs_kernels.hip).  All kernels are enqueued on `stream`.
 void launch_unbin(const Buffers &b, int nBinned, hipStream_t stream);
-// cap: particles staged in LDS per workgroup (0, 1024 or 2048; 0 reads
-// neighbours from global memory).  Every choice gives bit-identical results.
+// cap: particle positions staged in LDS per workgroup (0, 1024, 2048 or 4096;
+// 0 reads neighbours from global memory).  Every choice gives bit-identical
+// results.
 void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int nBinned, int lists,
                          int cap, hipStream_t stream);
 void launch_scan(const Buffers &b, uint32_t cells, hipStream_t stream);
 void launch_scatter_reorder(const Buffers &b, int n, hipStream_t stream);
 void launch_density(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int cap, hipStream_t stream);
 void launch_delta(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int cap, hipStream_t stream);
-constexpr int kTileCaps[3] = {0, 1024, 2048};
+constexpr int kNumTileCaps = 4;
+constexpr int kTileCaps[kNumTileCaps] = {0, 1024, 2048, 4096};
 void launch_gather_state(const Buffers &b, const DevScene &s0, int n, int nBinned, float2 *x, float2 *prev, float2 *v,
                          float2 *dens, float2 *pres, uint32_t *key, int32_t *scene, hipStream_t stream);
 void launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t stream);

@@ -372,9 +372,11 @@ class Simulation {
         ++s;
         continue;
       }
-      if (wantTune() && nsteps - s >= kTuneSteps) {
-        s += tune(dt, slots);
-        slots += kTuneSteps;
+      if (wantTune() && nsteps - s >= tuneSteps()) {
+        const uint32_t done = tune(dt, slots);
+        s += done;
+        slots += done;
+        stepsSinceTune_ = 0;
         continue;
       }
       const bool graphable = (options_ & NBS_OPT_HIP_GRAPH) && !timing_ && emitters_.empty() && lists_ &&
@@ -418,8 +420,8 @@ class Simulation {
 
   void setTuning(const nbs_tuning &t) {
     for (int k = 0; k < 3; ++k)
-      if (t.tile_cap[k] != 0 && t.tile_cap[k] != 1024 && t.tile_cap[k] != 2048)
-        throw Error(NBS_ERR_INVALID_ARGUMENT, "tile capacities are 0, 1024 or 2048");
+      if (t.tile_cap[k] != 0 && t.tile_cap[k] != 1024 && t.tile_cap[k] != 2048 && t.tile_cap[k] != 4096)
+        throw Error(NBS_ERR_INVALID_ARGUMENT, "tile capacities are 0, 1024, 2048 or 4096");
     tuning_ = t;
     tuned_ = false;
     stepsSinceTune_ = 0;
@@ -615,48 +617,67 @@ class Simulation {
   }
 
   // Autotuner: every tile capacity gives bit-identical results, so the
-  // candidates are timed on real steps.  Each of kTuneReps rounds runs one step
-  // per capacity (all pair kernels at that capacity) with HIP events; each
-  // kernel then keeps its fastest capacity (minimum over rounds).
+  // candidates are timed on real steps with HIP events.  The first tuning tries
+  // every capacity; later ones (every retune_interval steps, the fluid's
+  // packing changes as it moves) try the current choice and its neighbours.
+  // Each of kTuneReps rounds runs one step per candidate; each kernel keeps
+  // its fastest candidate (minimum over rounds).
   static constexpr uint32_t kTuneReps = 2;
-  static constexpr uint32_t kTuneSteps = kTuneReps * 3;
   static constexpr uint64_t kTuneMinParticles = 32768;
+  uint32_t tuneVariants() const { return tuned_ ? 3u : (uint32_t)kNumTileCaps; }
+  uint32_t tuneSteps() const { return kTuneReps * tuneVariants(); }
   bool wantTune() const {
     if (!tuning_.autotune || n_ < kTuneMinParticles || !lists_ || nBinned_ != n_) return false;
     return !tuned_ || (tuning_.retune_interval > 0 && stepsSinceTune_ >= tuning_.retune_interval);
   }
+  static int capIndex(int cap) {
+    for (int v = 0; v < kNumTileCaps; ++v)
+      if (kTileCaps[v] == cap) return v;
+    return kNumTileCaps - 1;
+  }
   uint32_t tune(float dt, uint32_t slot0) {
-    ensureEvents(slot0 + kTuneSteps);
+    const uint32_t V = tuneVariants(), steps = kTuneReps * V;
+    int cand[3][kNumTileCaps];
+    for (int k = 0; k < 3; ++k) {
+      if (!tuned_) {
+        for (uint32_t v = 0; v < V; ++v) cand[k][v] = kTileCaps[v];
+      } else {
+        const int c = capIndex(tuning_.tile_cap[k]);
+        cand[k][0] = kTileCaps[std::max(c - 1, 0)];
+        cand[k][1] = kTileCaps[c];
+        cand[k][2] = kTileCaps[std::min(c + 1, kNumTileCaps - 1)];
+      }
+    }
+    ensureEvents(slot0 + steps);
     for (uint32_t r = 0; r < kTuneReps; ++r) {
-      for (int v = 0; v < 3; ++v) {
-        for (int k = 0; k < 3; ++k) tuning_.tile_cap[k] = kTileCaps[v];
-        enqueueStep(dt, (int)(slot0 + r * 3 + v));
+      for (uint32_t v = 0; v < V; ++v) {
+        for (int k = 0; k < 3; ++k) tuning_.tile_cap[k] = cand[k][v];
+        enqueueStep(dt, (int)(slot0 + r * V + v));
         swapAfterStep();
       }
     }
-    NBS_HIP(hipEventSynchronize(events_[5 * (slot0 + kTuneSteps - 1) + 4]));
-    float best[3] = {3.0e38f, 3.0e38f, 3.0e38f};
-    int choice[3] = {kTileCaps[2], kTileCaps[2], kTileCaps[2]};
+    NBS_HIP(hipEventSynchronize(events_[5 * (slot0 + steps - 1) + 4]));
     const int evPair[3][2] = {{0, 1}, {2, 3}, {3, 4}};  // viscosity, density, relaxation
-    for (int v = 0; v < 3; ++v) {
-      float t[3] = {3.0e38f, 3.0e38f, 3.0e38f};
-      for (uint32_t r = 0; r < kTuneReps; ++r) {
-        const uint32_t sl = slot0 + r * 3 + v;
-        for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 3; ++k) {
+      float best = 3.0e38f;
+      int choice = cand[k][0];
+      for (uint32_t v = 0; v < V; ++v) {
+        float t = 3.0e38f;
+        for (uint32_t r = 0; r < kTuneReps; ++r) {
+          const uint32_t sl = slot0 + r * V + v;
           float ms = 0.0f;
           if (hipEventElapsedTime(&ms, events_[5 * sl + evPair[k][0]], events_[5 * sl + evPair[k][1]]) == hipSuccess)
-            t[k] = std::min(t[k], ms);
+            t = std::min(t, ms);
         }
+        if (t < best) best = t, choice = cand[k][v];
       }
-      for (int k = 0; k < 3; ++k)
-        if (t[k] < best[k]) best[k] = t[k], choice[k] = kTileCaps[v];
+      tuning_.tile_cap[k] = choice;
     }
-    for (int k = 0; k < 3; ++k) tuning_.tile_cap[k] = choice[k];
     tuned_ = true;
     stepsSinceTune_ = 0;
     ++tuneCount_;
     destroyGraph();
-    return kTuneSteps;
+    return steps;
   }
 
   void swapAfterStep() {
@@ -807,7 +828,7 @@ class Simulation {
   double kernelMs_[NBS_KERNEL_COUNT] = {};
   uint64_t kernelLaunches_[NBS_KERNEL_COUNT] = {};
 
-  nbs_tuning tuning_{1, 256, {2048, 2048, 2048}};
+  nbs_tuning tuning_{1, 64, {2048, 2048, 2048}};
   bool tuned_ = false;
   uint64_t stepsSinceTune_ = 0;
   uint64_t tuneCount_ = 0;

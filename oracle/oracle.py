@@ -66,6 +66,7 @@ def lib():
         L.orc_set_gravity.argtypes = [vp, f, f]
         L.orc_set_external_force.argtypes = [vp, f, f]
         L.orc_step.argtypes = [vp, f, i32]
+        L.orc_add_emitter.argtypes = [vp, i32, f, f, f, f, f, f, f, f, ctypes.c_uint64]
         L.orc_count.restype = i64
         L.orc_count.argtypes = [vp]
         L.orc_total_cells.restype = i64
@@ -174,6 +175,10 @@ class OracleSim:
             self.add_circle(s, cx, cy, r)
         self.add_particles(s, spec.positions)
         return s
+
+    def add_emitter(self, scene, pos, direction, radius, speed, rate, duration, seed=0):
+        lib().orc_add_emitter(self.h, scene, float(pos[0]), float(pos[1]), float(direction[0]), float(direction[1]),
+                              float(radius), float(speed), float(rate), float(duration), int(seed))
 
     def set_gravity(self, gx, gy):
         lib().orc_set_gravity(self.h, float(gx), float(gy))

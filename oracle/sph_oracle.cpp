@@ -16,6 +16,23 @@ static inline float msSince(Clock::time_point t0) {
   return (float)std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now() - t0).count() * (1.0f / 1000000);
 }
 
+// SplitMix64 uniform in [-1, 1) with 24-bit resolution: the seeded stand-in
+// for the reference's Vec2RandomDirection() jitter (vecmath.h:317-322).
+static inline float splitmixUniform(uint64_t seed, uint64_t counter) {
+  uint64_t z = seed + (counter + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (float)(uint32_t)(z >> 40) * (1.0f / 8388608.0f) - 1.0f;
+}
+
+struct Emitter {  // demo4.h:123-135
+  int scene;
+  float px, py, dx, dy, radius, speed, rate, duration, elapsed, totalElapsed;
+  bool active;
+  uint64_t seed, draws;
+};
+
 // ---------------------------------------------------------------------------
 // Abstract simulation used by the C API.
 // ---------------------------------------------------------------------------
@@ -26,6 +43,7 @@ struct SimBase {
   virtual int64_t addParticles(int scene, const float *pos, const float *force, int64_t n) = 0;
   virtual void setGravity(float gx, float gy) = 0;
   virtual void setExternalForce(float fx, float fy) = 0;
+  virtual void addEmitter(const Emitter &e) = 0;
   virtual void step(float dt, int nsteps) = 0;
   virtual int64_t count() const = 0;
   virtual int64_t totalCells() const = 0;
@@ -122,6 +140,38 @@ struct World {
   }
 };
 
+// demo4.cpp:257-284 UpdateEmitter, in FP32 like the reference; emitted particles
+// are added through the simulation's AddParticle.
+template <typename AddFn>
+static void runEmitter(Emitter &e, float deltaTime, float spacing, float h, AddFn &&addParticle) {
+  const float invDeltaTime = 1.0f / deltaTime;
+  if (!e.active) return;
+  const float rate = 1.0f / e.rate;
+  e.elapsed += deltaTime;
+  e.totalElapsed += deltaTime;
+  if (e.elapsed >= rate) {
+    e.elapsed = 0;
+    const float ax = (e.dx * e.speed) * invDeltaTime, ay = (e.dy * e.speed) * invDeltaTime;
+    const float dirx = -1.0f * e.dy, diry = 1.0f * e.dx;  // vecmath.h:304-306 Vec2Cross(1, d)
+    const int count = (int)std::floor(e.radius / spacing);
+    const float offx = ((dirx * (float)count) * spacing) * 0.5f;
+    const float offy = ((diry * (float)count) * spacing) * 0.5f;
+    for (int index = 0; index < count; ++index) {
+      float px = (dirx * (float)index) * spacing, py = (diry * (float)index) * spacing;
+      px = ((dirx * spacing) * 0.5f) + px;
+      py = ((diry * spacing) * 0.5f) + py;
+      px = (e.px - offx) + px;
+      py = (e.py - offy) + py;
+      const float jx = (splitmixUniform(e.seed, e.draws) * h) * 0.01f;
+      const float jy = (splitmixUniform(e.seed, e.draws + 1) * h) * 0.01f;
+      e.draws += 2;
+      const float pos[2] = {jx + px, jy + py}, force[2] = {ax, ay};
+      addParticle(e.scene, pos, force);
+    }
+  }
+  if (e.totalElapsed >= e.duration) e.active = false;
+}
+
 // ---------------------------------------------------------------------------
 // Reference order (demo4.cpp), serial or pooled.
 // ---------------------------------------------------------------------------
@@ -163,8 +213,10 @@ struct GSSim : SimBase {
   std::vector<std::vector<size_t>> cells;  // demo4.h:118-121, capacity made dynamic
   Stats st;
   std::unique_ptr<Pool> pool;  // null: serial order
+  std::vector<Emitter> emitters;
+  float spacing;
 
-  GSSim(const ParamsF &pf, int threads) {
+  GSSim(const ParamsF &pf, int threads) : spacing(pf.h * 0.5f) {
     w.p = Params<T>::from(pf);
     if (threads > 0) pool.reset(new Pool(threads));
     resetStats();
@@ -184,6 +236,7 @@ struct GSSim : SimBase {
   int addBody(int scene, int type, const float *d, int nv) override { return w.addBody(scene, type, d, nv); }
   void setGravity(float gx, float gy) override { w.gravity = {T(gx), T(gy)}; }
   void setExternalForce(float fx, float fy) override { w.externalForce = {T(fx), T(fy)}; }
+  void addEmitter(const Emitter &e) override { emitters.push_back(e); }
   int64_t count() const override { return (int64_t)pd.size(); }
   int64_t totalCells() const override { return w.cells; }
 
@@ -309,9 +362,12 @@ struct GSSim : SimBase {
   // demo4.cpp:286-451 Update
   void update(T dt) {
     const T invDt = T(1) / dt;
+    Clock::time_point t0 = Clock::now();  // demo4.cpp:290-299 emitters
+    for (Emitter &e : emitters)
+      runEmitter(e, (float)dt, spacing, (float)w.p.h,
+                 [&](int sc, const float *pos, const float *force) { addParticles(sc, pos, force, 1); });
     const int64_t n = (int64_t)pd.size();
-    Clock::time_point t0;
-    st.time[0] = 0.0f;  // emitters (none)
+    st.time[0] = msSince(t0);
 
     t0 = Clock::now();  // demo4.cpp:301-312 integrate
     for (int64_t i = 0; i < n; ++i) {
@@ -414,8 +470,10 @@ struct JacobiSim : SimBase {
   int64_t nBinned = 0;  // particles present at the last binning
   Stats st;
   std::unique_ptr<Pool> pool;
+  std::vector<Emitter> emitters;
+  float spacing;
 
-  JacobiSim(const ParamsF &pf, int threads) {
+  JacobiSim(const ParamsF &pf, int threads) : spacing(pf.h * 0.5f) {
     w.p = Params<T>::from(pf);
     if (threads > 1) pool.reset(new Pool(threads));
     resetStats();
@@ -434,6 +492,7 @@ struct JacobiSim : SimBase {
   int addBody(int sc, int type, const float *d, int nv) override { return w.addBody(sc, type, d, nv); }
   void setGravity(float gx, float gy) override { w.gravity = {T(gx), T(gy)}; }
   void setExternalForce(float fx, float fy) override { w.externalForce = {T(fx), T(fy)}; }
+  void addEmitter(const Emitter &e) override { emitters.push_back(e); }
   int64_t count() const override { return (int64_t)x.size(); }
   int64_t totalCells() const override { return w.cells; }
 
@@ -500,9 +559,12 @@ struct JacobiSim : SimBase {
 
   void update(T dt) {
     const T invDt = T(1) / dt;
+    Clock::time_point t0 = Clock::now();  // demo4.cpp:290-299 emitters
+    for (Emitter &e : emitters)
+      runEmitter(e, (float)dt, spacing, (float)w.p.h,
+                 [&](int sc, const float *pos, const float *force) { addParticles(sc, pos, force, 1); });
     const int64_t n = (int64_t)x.size();
-    Clock::time_point t0;
-    st.time[0] = 0.0f;
+    st.time[0] = msSince(t0);
 
     t0 = Clock::now();  // demo4.cpp:301-312 integrate
     const V2<T> gPlusE = vadd(w.gravity, w.externalForce);
@@ -660,6 +722,11 @@ int64_t orc_add_particles(void *h, int scene, const float *pos, const float *for
 void orc_set_gravity(void *h, float gx, float gy) { ((SimBase *)h)->setGravity(gx, gy); }
 void orc_set_external_force(void *h, float fx, float fy) { ((SimBase *)h)->setExternalForce(fx, fy); }
 void orc_step(void *h, float dt, int nsteps) { ((SimBase *)h)->step(dt, nsteps); }
+// spacing: SPHParameters::particleSpacing (demo4.cpp:258); the constructor uses h/2.
+void orc_add_emitter(void *h, int scene, float px, float py, float dx, float dy, float radius, float speed,
+                     float rate, float duration, uint64_t seed) {
+  ((SimBase *)h)->addEmitter(Emitter{scene, px, py, dx, dy, radius, speed, rate, duration, 0.0f, 0.0f, true, seed, 0});
+}
 int64_t orc_count(void *h) { return ((SimBase *)h)->count(); }
 int64_t orc_total_cells(void *h) { return ((SimBase *)h)->totalCells(); }
 void orc_get_state(void *h, double *x, double *prev, double *v, double *dens, double *pres, uint32_t *key) {

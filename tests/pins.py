@@ -290,6 +290,31 @@ def pin_swap_remove():
         raise AssertionError(f"swap-remove order {order[start[k0]:start[k0 + 1]].tolist()}")
 
 
+def pin_emitter(precision=32):
+    # demo4.cpp:257-284 with exact binary timing: dt = 1/128 and rate = 32 emit a
+    # column every 4 steps; duration 0.25 s stops after step 32.  A column holds
+    # floor(radius / spacing) particles spaced along the left perpendicular of
+    # the direction, centred on the emitter, each within 0.01h of its lattice point.
+    for mode in (O.JACOBI, O.GS):
+        s = _sim(precision, mode, SPHParams(k=0.0, k_near=0.0), 80.0, 80.0)
+        s.add_emitter(0, (-2.0, 3.0), (1.0, 0.0), 2.0, 0.0, 32.0, 0.25, seed=5)
+        s.step(1.0 / 128.0, 3)
+        if s.n != 0:
+            raise AssertionError(f"emitted before 1/rate: {s.n}")
+        s.step(1.0 / 128.0, 1)
+        st = s.state()
+        if s.n != 4:
+            raise AssertionError(f"first column has {s.n} particles")
+        # spacing h/2, dir (1, 0): column along +y, offsets (-0.75, -0.25, 0.25, 0.75), then one
+        # step of predict moves nothing (zero speed, zero gravity)
+        want = np.array([[-2.0, 3.0 + o] for o in (-0.75, -0.25, 0.25, 0.75)])
+        if np.abs(st["x"].astype(np.float64) - want).max() > 0.0100001:
+            raise AssertionError(f"column positions {st['x'].tolist()}")
+        s.step(1.0 / 128.0, 60)
+        if s.n != 4 * 8:
+            raise AssertionError(f"{s.n} particles after the emitter stopped, want 32")
+
+
 ALL_PINS = [pin_cell_keys, pin_pair_density, pin_strict_cutoff, pin_self_pair, pin_lattice_density, pin_pressure, pin_pair_delta,
             pin_pair_viscosity, pin_plane_collision, pin_circle_collision, pin_segment_collision,
-            pin_polygon_collision, pin_free_fall, pin_momentum, pin_neighbor_sets, pin_grid_order, pin_swap_remove]
+            pin_polygon_collision, pin_free_fall, pin_momentum, pin_neighbor_sets, pin_grid_order, pin_swap_remove, pin_emitter]

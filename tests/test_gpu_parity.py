@@ -178,7 +178,8 @@ def test_dam_break_1m_three_steps_bit_exact():
     assert_bit_exact(sim.get_state(), ref.state(), "1M step 3: ")
 
 
-@pytest.mark.parametrize("caps", [(0, 0, 0), (1024, 1024, 1024), (2048, 2048, 2048), (0, 2048, 1024)])
+@pytest.mark.parametrize("caps", [(0, 0, 0), (1024, 1024, 1024), (2048, 2048, 2048), (4096, 4096, 4096),
+                                  (0, 4096, 1024)])
 def test_tile_capacities_bit_exact(caps):
     spec = dam_break_4k()
     sim = gpu_for([spec])
@@ -199,4 +200,4 @@ def test_autotuned_batched_run_bit_exact():
     ref.step(DT, 60)
     assert_bit_exact(sim.get_state(), ref.state(), "autotuned batched: ")
     t = sim.get_tuning()
-    assert all(c in (0, 1024, 2048) for c in t["tile_cap"])
+    assert all(c in (0, 1024, 2048, 4096) for c in t["tile_cap"])
