@@ -182,6 +182,11 @@ nbs_status nbs_get_particle_count(nbs_sim *sim, uint64_t *count);
  * the scene's first cell) of the last binning; scene holds n scene indices. */
 nbs_status nbs_get_state(nbs_sim *sim, float *x, float *prev, float *v, float *density, float *pressure,
                          uint32_t *cell_key, int32_t *scene);
+/* Particle colours of Demo4::Render (demo4.cpp:521-524, SPHGetParticleColor
+ * sph.h:683-695): rgba holds 4*n floats in particle-index order, r = clamp(
+ * pressure / -10), g = clamp(density / rest_density), b = clamp(|v| / 10), a = 1. */
+nbs_status nbs_get_colors(nbs_sim *sim, float *rgba);
+
 /* Restores positions and velocities (particle-index order, 2*n floats each,
  * n = current particle count).  prev is set to x and the next step runs
  * without viscosity, as the first step after AddParticle does. */

@@ -96,6 +96,7 @@ SIGNATURES = {
     "nbs_get_particle_count": [vp, P(c_uint64)],
     "nbs_get_state": [vp, vp, vp, vp, vp, vp, vp, vp],
     "nbs_set_state": [vp, vp, vp],
+    "nbs_get_colors": [vp, vp],
     "nbs_get_stats": [vp, P(NbsStats)],
     "nbs_get_kernel_times": [vp, vp, vp],
     "nbs_set_timing": [vp, c_int32],

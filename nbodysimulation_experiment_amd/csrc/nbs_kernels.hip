@@ -745,6 +745,22 @@ __global__ __launch_bounds__(kBlock) void k_gather_state(Buffers b, DevScene s0,
   }
 }
 
+// sph.h:683-695 SPHGetParticleColor, in particle-index order.  The clamps
+// keep std::min / std::max's comparison order (NaN passes through).
+__global__ __launch_bounds__(kBlock) void k_colors(Buffers b, float rho0, int n, float4 *rgba) {
+  const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (i >= n) return;
+  const float2 d = b.dens[i], pr = b.pres[i], v = b.v[i];
+  const float r = pr.x / (-10.0f);
+  const float g = d.x / rho0;
+  const float bl = sqrtf(v.x * v.x + v.y * v.y) / 10.0f;  // vecmath.h:267-270 Vec2Length
+  auto clamp01 = [](float c) {
+    const float m = (1.0f < c) ? 1.0f : c;  // std::min(c, 1.0f)
+    return (m < 0.0f) ? 0.0f : m;           // std::max(m, 0.0f)
+  };
+  rgba[b.id[i]] = make_float4(clamp01(r), clamp01(g), clamp01(bl), 1.0f);
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *p, uint32_t value, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i < n) p[i] = value;
@@ -933,6 +949,11 @@ void launch_gather_state(const Buffers &b, const DevScene &s0, int n, int nBinne
   if (n <= 0) return;
   hipLaunchKernelGGL(k_gather_state, dim3(grid_for(n)), dim3(kBlock), 0, s, b, s0, n, nBinned, x, prev, v, dens, pres,
                      key, scene);
+}
+
+void launch_colors(const Buffers &b, float rho0, int n, float4 *rgba, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_colors, dim3(grid_for(n)), dim3(kBlock), 0, s, b, rho0, n, rgba);
 }
 
 void launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t s) {

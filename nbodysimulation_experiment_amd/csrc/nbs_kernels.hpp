@@ -87,6 +87,7 @@ constexpr int kNumTileCaps = 4;
 constexpr int kTileCaps[kNumTileCaps] = {0, 1024, 2048, 4096};
 void launch_gather_state(const Buffers &b, const DevScene &s0, int n, int nBinned, float2 *x, float2 *prev, float2 *v,
                          float2 *dens, float2 *pres, uint32_t *key, int32_t *scene, hipStream_t stream);
+void launch_colors(const Buffers &b, float rho0, int n, float4 *rgba, hipStream_t stream);
 void launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t stream);
 void launch_diagnostics(const Buffers &b, const DevParams &p, int n, double *partials, double *out,
                         hipStream_t stream);

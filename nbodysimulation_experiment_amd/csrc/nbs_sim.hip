@@ -258,6 +258,15 @@ class Simulation {
     if (scene) NBS_HIP(hipMemcpyAsync(scene, stScene_.p, n_ * 4, hipMemcpyDeviceToHost, stream_));
     NBS_HIP(hipStreamSynchronize(stream_));
   }
+  // sph.h:683-695 per-particle colour, as demo4.cpp:521-524 computes it for Render.
+  void getColors(float *rgba) {
+    if (n_ == 0) return;
+    if (stColor_.n < n_) stColor_.alloc(cap_);
+    launch_colors(buf_, params_.rest_density, (int)n_, stColor_.p, stream_);
+    NBS_HIP(hipGetLastError());
+    NBS_HIP(hipMemcpyAsync(rgba, stColor_.p, n_ * sizeof(float4), hipMemcpyDeviceToHost, stream_));
+    NBS_HIP(hipStreamSynchronize(stream_));
+  }
   void setState(const float *x, const float *v) {
     if (n_ == 0) return;
     if (!x || !v) throw Error(NBS_ERR_INVALID_ARGUMENT, "x and v are required");
@@ -815,6 +824,7 @@ class Simulation {
   DevBuf<float2> stX_, stPrev_, stV_, stDens_, stPres_;
   DevBuf<uint32_t> stKey_;
   DevBuf<int32_t> stScene_;
+  DevBuf<float4> stColor_;
   DevBuf<double> diagPartials_, diagOut_;
 
   uint64_t n_ = 0, nBinned_ = 0, cap_ = 0;
@@ -1033,6 +1043,12 @@ nbs_status nbs_get_state(nbs_sim *sim, float *x, float *prev, float *v, float *d
                          uint32_t *cell_key, int32_t *scene) {
   NBS_CHECK_SIM(sim);
   return guarded([&] { sim->impl->getState(x, prev, v, density, pressure, cell_key, scene); });
+}
+
+nbs_status nbs_get_colors(nbs_sim *sim, float *rgba) {
+  NBS_CHECK_SIM(sim);
+  if (!rgba) return NBS_ERR_INVALID_ARGUMENT;
+  return guarded([&] { sim->impl->getColors(rgba); });
 }
 
 nbs_status nbs_set_state(nbs_sim *sim, const float *x, const float *v) {

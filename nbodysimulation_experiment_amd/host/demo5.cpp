@@ -126,10 +126,20 @@ const ParticleData *ParticleSimulation::GetParticleDatas() {
   return mirror_.data();
 }
 
+const float *ParticleSimulation::GetParticleColors() {
+  colors_.resize(4 * GetParticleCount());
+  if (!colors_.empty()) check(nbs_get_colors(sim_, colors_.data()), "nbs_get_colors");
+  return colors_.data();
+}
+
 // Drawing belongs to the host application (demo4.cpp:453-532 pushes the grid,
 // bodies and the particle vertex array into the command buffer).  This method
-// refreshes the 48-byte mirror that vertex array points into.
-void ParticleSimulation::Render(Render::CommandBuffer *, const float) { refreshMirror(); }
+// refreshes the 48-byte mirror that vertex array points into and the colours
+// computed on the GPU.
+void ParticleSimulation::Render(Render::CommandBuffer *, const float) {
+  refreshMirror();
+  GetParticleColors();
+}
 
 void ParticleSimulation::AddExternalForces(const Vec2f &force) {
   check(nbs_add_external_force(sim_, force.x, force.y), "nbs_add_external_force");

@@ -84,6 +84,7 @@ class ParticleSimulation : public BaseSimulation {
   // Extensions.
   void UpdateSteps(float deltaTime, uint32_t steps);  // several steps per call
   const ParticleData *GetParticleDatas();              // refreshed 48-byte mirror
+  const float *GetParticleColors();                    // RGBA per particle (demo4.cpp:521-524)
   nbs_sim *Handle() { return sim_; }
   void SetSeed(uint64_t seed) { seed_ = seed; }         // jitter seed of the next AddVolume/AddEmitter
 
@@ -96,6 +97,7 @@ class ParticleSimulation : public BaseSimulation {
   SPHStatistics stats_{};
   float collisionRadius_, collisionMargin_;
   std::vector<ParticleData> mirror_;
+  std::vector<float> colors_;
   std::vector<float> scratch_;
   bool mirrorDirty_ = true;
   bool multiThreading_ = false;

@@ -238,6 +238,12 @@ class ParticleSimulation:
              _ptr(out["pressure"]), _ptr(out["key"]), _ptr(out["scene"]))
         return out
 
+    def get_colors(self) -> np.ndarray:
+        """Per-particle RGBA of the reference's Render (sph.h:683-695), particle-index order."""
+        out = np.zeros((self.GetParticleCount(), 4), np.float32)
+        call("nbs_get_colors", self._h, _ptr(out))
+        return out
+
     def set_state(self, x, v):
         x, v = _f32(x), _f32(v)
         n = self.GetParticleCount()

@@ -86,6 +86,7 @@ def lib():
         L.orc_kat_viscosity.argtypes = [i32, P(ParamsF), vp, vp, vp, vp, i64, vp]
         L.orc_kat_collide.argtypes = [i32, P(ParamsF), i32, vp, i32, vp, i64, vp]
         L.orc_kat_cell.argtypes = [i32, f, f, f, vp, i64, vp]
+        L.orc_kat_color.argtypes = [f, vp, vp, vp, i64, vp]
         _lib = L
     return _lib
 
@@ -287,4 +288,12 @@ def kat_cell(width, height, cell_size, pos, precision=32):
     pos = _f32(pos, (-1, 2))
     out = np.zeros((pos.shape[0], 3), np.int32)
     lib().orc_kat_cell(precision, float(width), float(height), float(cell_size), _ptr(pos), pos.shape[0], _ptr(out))
+    return out
+
+
+def kat_color(rest_density, density, pressure, vel):
+    density, pressure = _f32(density).reshape(-1), _f32(pressure).reshape(-1)
+    vel = _f32(vel, (-1, 2))
+    out = np.zeros((density.size, 4), np.float32)
+    lib().orc_kat_color(float(rest_density), _ptr(density), _ptr(pressure), _ptr(vel), density.size, _ptr(out))
     return out

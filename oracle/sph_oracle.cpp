@@ -825,6 +825,20 @@ void orc_kat_collide(int prec, const ParamsF *pf, int type, const float *data, i
     }
   });
 }
+// sph.h:683-695 SPHGetParticleColor (FP32, std::min / std::max order).
+void orc_kat_color(float restDensity, const float *density, const float *pressure, const float *vel, int64_t n,
+                   float *rgba) {
+  for (int64_t k = 0; k < n; ++k) {
+    float r = pressure[k] / (-10.0f);
+    float g = density[k] / (restDensity);
+    float b = vlen(V2<float>{vel[2 * k], vel[2 * k + 1]}) / 10.0f;
+    rgba[4 * k + 0] = std::max(std::min(r, 1.0f), 0.0f);
+    rgba[4 * k + 1] = std::max(std::min(g, 1.0f), 0.0f);
+    rgba[4 * k + 2] = std::max(std::min(b, 1.0f), 0.0f);
+    rgba[4 * k + 3] = 1.0f;
+  }
+}
+
 void orc_kat_cell(int prec, float width, float height, float cellSize, const float *pos, int64_t n, int32_t *out) {
   ORC_DISPATCH(prec, {
     World<T> w;

@@ -315,6 +315,17 @@ def pin_emitter(precision=32):
             raise AssertionError(f"{s.n} particles after the emitter stopped, want 32")
 
 
+def pin_color():
+    # sph.h:683-695: r = clamp(P / -10), g = clamp(rho / rho0), b = clamp(|v| / 10), a = 1
+    dens = np.array([0.0, 5.0, 10.0, 25.0, 7.5], np.float32)
+    pres = np.array([-20.0, -5.0, 0.0, 3.0, -2.5], np.float32)
+    vel = np.array([[0, 0], [3, 4], [60, 80], [0.5, 0], [6, 8]], np.float32)
+    got = O.kat_color(10.0, dens, pres, vel)
+    want = [[1.0, 0.0, 0.0, 1.0], [0.5, 0.5, 0.5, 1.0], [0.0, 1.0, 1.0, 1.0], [0.0, 1.0, 0.05, 1.0],
+            [0.25, 0.75, 1.0, 1.0]]
+    _close(got, want, 1e-6, 1e-7, "particle colours")
+
+
 ALL_PINS = [pin_cell_keys, pin_pair_density, pin_strict_cutoff, pin_self_pair, pin_lattice_density, pin_pressure, pin_pair_delta,
             pin_pair_viscosity, pin_plane_collision, pin_circle_collision, pin_segment_collision,
-            pin_polygon_collision, pin_free_fall, pin_momentum, pin_neighbor_sets, pin_grid_order, pin_swap_remove, pin_emitter]
+            pin_polygon_collision, pin_free_fall, pin_momentum, pin_neighbor_sets, pin_grid_order, pin_swap_remove, pin_emitter, pin_color]

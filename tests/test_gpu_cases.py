@@ -175,3 +175,14 @@ def test_fp64_diagnostics_match_host_sums():
     np.testing.assert_allclose(d["mean_density"], rho.mean(), rtol=1e-12)
     np.testing.assert_allclose(d["max_density"], rho.max(), rtol=0)
     np.testing.assert_allclose(d["mean_abs_density_error"], np.abs(rho - 10.0).mean() / 10.0, rtol=1e-12)
+
+
+def test_particle_colors_bit_exact():
+    from oracle.oracle import kat_color
+    spec = _box_scene(seed=10)
+    sim = gpu_for([spec])
+    sim.Update(DT, 80)
+    st = sim.get_state()
+    got = sim.get_colors()
+    want = kat_color(10.0, st["density"][:, 0], st["pressure"][:, 0], st["v"])
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
