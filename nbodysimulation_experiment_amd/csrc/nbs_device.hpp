@@ -1,0 +1,229 @@
+// Device helpers shared by the SPH step kernels and the learned relaxation:
+// cell keys, scene lookup, correctly rounded arithmetic, boundary collisions,
+// candidate row ranges.
+#pragma once
+
+#include "nbs_kernels.hpp"
+
+namespace nbs {
+
+// ---------------------------------------------------------------------------
+// Helpers
+// ---------------------------------------------------------------------------
+
+// Workgroup index with consecutive tiles kept on one XCD.  The dispatcher
+// places workgroup b on XCD (b mod 8); this remap gives XCD x a contiguous
+// range of tiles so neighbouring cells share that XCD's L2.
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned b = blockIdx.x, nb = gridDim.x;
+  const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u, loc = b >> 3;
+  return (x < r) ? x * (q + 1u) + loc : r * (q + 1u) + (x - r) * q + loc;
+}
+
+// sph.h:450-463 per axis.  Out-of-range values are clamped in floating point so
+// NaN and values beyond the int range are defined (NaN -> 0).
+__device__ __forceinline__ int cell_coord(float p, float half, float cs, int n) {
+  const float f = (p + half) / cs;
+  if (!(f >= 0.0f)) return 0;
+  if (f >= (float)n) return n - 1;
+  return (int)f;
+}
+
+__device__ __forceinline__ int scene_of_key(const uint32_t *base, int n, uint32_t key) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (base[mid] <= key) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool MULTI>
+__device__ __forceinline__ DevScene scene_of_slot(const Buffers &b, const DevScene &s0, uint32_t key) {
+  if (!MULTI) return s0;
+  return b.scenes[scene_of_key(b.sceneBase, b.nScenes, key)];
+}
+
+// Correctly rounded FP32 reciprocal and square root in fewer instructions than
+// the general IEEE lowering, for the operand ranges of the pair kernels.
+//
+// rcp_rn: v_rcp_f32 (within 1 ulp) refined by one Newton step with FMA
+// (Markstein): e = 1 - x*y exactly, y' = y + e*y.  Used for x in [2^-60, 2^60];
+// other operands take the IEEE division.
+// sqrt_rn: v_sqrt_f32 followed by the residual test of the neighbouring floats
+// (s - ulp, s + ulp): x - s'*s computed with one FMA decides the rounding.  Used
+// for x == 0 and x in [2^-96, 2^100]; other operands take sqrtf.
+// Both are checked against the correctly rounded results for every mantissa of
+// several binades by the device arithmetic self-test.
+__device__ __forceinline__ float rcp_rn(float x) {
+  if (!(x >= 0x1p-60f && x <= 0x1p60f)) return 1.0f / x;
+  const float y = __builtin_amdgcn_rcpf(x);
+  const float e = __builtin_fmaf(-x, y, 1.0f);
+  return __builtin_fmaf(e, y, y);
+}
+
+__device__ __forceinline__ float sqrt_rn(float x) {
+  if (!((x >= 0x1p-96f && x <= 0x1p100f) || x == 0.0f)) return sqrtf(x);
+  float s = __builtin_amdgcn_sqrtf(x);
+  const float dn = __uint_as_float(__float_as_uint(s) - 1u);
+  const float up = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rdn = __builtin_fmaf(-dn, s, x);
+  const float rup = __builtin_fmaf(-up, s, x);
+  if (rdn <= 0.0f) s = dn;
+  if (rup > 0.0f) s = up;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Boundary collisions (sph.h:514-681), bodies of the scene in insertion order
+// (demo4.cpp:414-441).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void normalize2(float vx, float vy, float &nx, float &ny) {
+  float l = sqrtf(vx * vx + vy * vy);  // vecmath.h:267-280
+  if (l == 0.0f) l = 1.0f;
+  const float inv = 1.0f / l;
+  nx = vx * inv;
+  ny = vy * inv;
+}
+
+__device__ __forceinline__ void solve_plane(const DevParams &p, float &x, float &y, float nx, float ny, float d) {
+  const float ppx = nx * d, ppy = ny * d;                 // sph.h:515
+  const float dx = x - ppx, dy = y - ppy;                 // sph.h:517
+  const float proj = dx * nx + dy * ny;                   // sph.h:518
+  if (proj <= p.R) {                                      // sph.h:519
+    const float pen = p.R - proj;                         // sph.h:520
+    x = (nx * pen) + x;                                   // sph.h:521
+    y = (ny * pen) + y;
+  }
+}
+
+__device__ __forceinline__ void solve_circle(const DevParams &p, float &x, float &y, float cx, float cy, float r) {
+  const float both = r + p.R;                             // sph.h:527
+  const float dx = x - cx, dy = y - cy;                   // sph.h:529
+  const float d2 = dx * dx + dy * dy;                     // sph.h:530
+  if (d2 <= both * both) {                                // sph.h:531
+    if (fabsf(d2) > 0.0f) {                               // sph.h:534
+      const float dist = sqrtf(d2);
+      const float inv = 1.0f / dist;
+      const float nx = dx * inv, ny = dy * inv;           // sph.h:536
+      const float pen = both - dist;                      // sph.h:537
+      x = (nx * pen) + x;                                 // sph.h:538
+      y = (ny * pen) + y;
+    }
+  }
+}
+
+__device__ __forceinline__ float dot2(float ax, float ay, float bx, float by) { return ax * bx + ay * by; }
+
+__device__ __noinline__ void solve_segment(const DevParams &p, float &x, float &y, float ax, float ay, float bx,
+                                           float by) {
+  const float both = p.margin + p.R;                      // sph.h:545
+  const float ex = bx - ax, ey = by - ay;                 // sph.h:548
+  const float u = dot2(ex, ey, bx - x, by - y);           // sph.h:549
+  const float v = dot2(ex, ey, x - ax, y - ay);           // sph.h:550
+  float clx, cly, nx, ny;
+  if (v <= 0.0f) {                                        // sph.h:555-563
+    clx = ax; cly = ay;
+    const float dx = x - clx, dy = y - cly;
+    if (dot2(dx, dy, dx, dy) > both * both) return;
+    normalize2(x - clx, y - cly, nx, ny);
+  } else if (u <= 0.0f) {                                 // sph.h:564-572
+    clx = bx; cly = by;
+    const float dx = x - clx, dy = y - cly;
+    if (dot2(dx, dy, dx, dy) > both * both) return;
+    normalize2(x - clx, y - cly, nx, ny);
+  } else {                                                // sph.h:573-591
+    const float den = dot2(ex, ey, ex, ey);
+    const float inv = 1.0f / den;
+    clx = ((ax * u) + (bx * v)) * inv;
+    cly = ((ay * u) + (by * v)) * inv;
+    const float dx = x - clx, dy = y - cly;
+    if (dot2(dx, dy, dx, dy) > both * both) return;
+    float px = -ey, py = ex;
+    if (dot2(px, py, x - ax, y - ay) < 0.0f) { px = -px; py = -py; }
+    normalize2(px, py, nx, ny);
+  }
+  const float dist = dot2(nx, ny, x - clx, y - cly);      // sph.h:593-594
+  const float pen = both - dist;                          // sph.h:595
+  x = (nx * pen) + x;                                     // sph.h:596
+  y = (ny * pen) + y;
+}
+
+__device__ __noinline__ void solve_polygon(const DevParams &p, float &x, float &y, int nv, const float2 *verts) {
+  // sph.h:600-672 FindMTVCirclePolygon, sph.h:674-681 SPHSolvePolygonCollision
+  const float radius = p.margin + p.R;
+  int edge = 0;
+  float nX = 0.0f, nY = 0.0f, sep = -3.402823466e+38f;
+  for (int k = 0; k < nv; ++k) {
+    const float2 a = verts[k];
+    const float2 bb = verts[(k + 1) % nv];
+    const float ex = bb.x - a.x, ey = bb.y - a.y;
+    float cx, cy;
+    normalize2(1.0f * ey, -1.0f * ex, cx, cy);  // vecmath.h:299-301 Vec2Cross(v, 1)
+    const float s = dot2(cx, cy, x - a.x, y - a.y);
+    if (s > radius) return;
+    if (s > sep) { nX = cx; nY = cy; sep = s; edge = k; }
+  }
+  const float2 v1 = verts[edge];
+  const float2 v2 = verts[(edge + 1) % nv];
+  float mx, my;
+  if (sep < 1.19209290e-07f) {                            // sph.h:626 FLT_EPSILON
+    const float pen = radius - sep;
+    mx = nX * pen; my = nY * pen;
+  } else {
+    const float u1 = dot2(x - v1.x, y - v1.y, v2.x - v1.x, v2.y - v1.y);
+    const float u2 = dot2(x - v2.x, y - v2.y, v1.x - v2.x, v1.y - v2.y);
+    if (u1 <= 0.0f || u2 <= 0.0f) {
+      const float2 c = (u1 <= 0.0f) ? v1 : v2;
+      // vecmath.h:292-296: the reference's "distance squared" is ((bx-ax)*(by-ay))^2
+      const float f = (c.x - x) * (c.y - y);
+      if (f * f > radius * radius) return;
+      const float dx = x - c.x, dy = y - c.y;
+      float ux, uy;
+      normalize2(dx, dy, ux, uy);
+      const float pen = radius - dot2(ux, uy, dx, dy);
+      mx = ux * pen; my = uy * pen;
+    } else {
+      // vecmath.h:228-231 ScalarLerp(a, 0.5, b) = (1 - 0.5) * a + 0.5 * b
+      const float fx = (1.0f - 0.5f) * v1.x + 0.5f * v2.x;
+      const float fy = (1.0f - 0.5f) * v1.y + 0.5f * v2.y;
+      const float s = dot2(x - fx, y - fy, nX, nY);
+      if (s > radius) return;
+      const float pen = radius - s;
+      mx = nX * pen; my = nY * pen;
+    }
+  }
+  x = x + mx;
+  y = y + my;
+}
+
+__device__ __forceinline__ void collide(const Buffers &b, const DevParams &p, const DevScene &sc, float &x, float &y) {
+  for (int k = sc.bodyBegin; k < sc.bodyEnd; ++k) {
+    const DevBody bd = b.bodies[k];
+    switch (bd.type) {
+      case BODY_PLANE: solve_plane(p, x, y, bd.a0, bd.a1, bd.a2); break;
+      case BODY_CIRCLE: solve_circle(p, x, y, bd.a0, bd.a1, bd.a2); break;
+      case BODY_SEGMENT: solve_segment(p, x, y, bd.a0, bd.a1, bd.a2, bd.a3); break;
+      case BODY_POLYGON: solve_polygon(p, x, y, bd.nverts, b.verts + bd.vertOffset); break;
+      default: break;
+    }
+  }
+}
+
+// Candidate ranges of a particle in cell (cx, cy): the 3x3 cells dy = -1..1
+// (outer), dx = -1..1 (inner), restricted to the grid (demo4.cpp:188-203).  The
+// three cells of one row have consecutive keys, so each row is one contiguous
+// range of sorted slots.
+struct RowRange {
+  uint32_t j0, j1;
+};
+
+__device__ __forceinline__ RowRange row_range(const uint32_t *start, const DevScene &sc, int cx, int cy, int dy) {
+  const int yy = cy + dy;
+  if (yy < 0 || yy >= sc.ny) return {0u, 0u};
+  const int xlo = max(cx - 1, 0), xhi = min(cx + 1, sc.nx - 1);
+  const uint32_t row = sc.cellBase + (uint32_t)(yy * sc.nx);
+  return {start[row + xlo], start[row + xhi + 1]};
+}
+
+}  // namespace nbs
