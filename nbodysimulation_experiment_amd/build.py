@@ -51,24 +51,37 @@ def build_library(verbose: bool = False) -> str:
 
 
 DRIVER = os.path.join(ROOT, "tests", "cpp", "demo5_driver")
+BENCHMARK = os.path.join(HERE, "nbs_benchmark")
+HOST = os.path.join(HERE, "host")
+HOST_SOURCES = [os.path.join(HOST, f) for f in ("demo5.cpp", "scenarios.cpp", "benchmark.cpp")]
 
 
-def build_host_driver(verbose: bool = False) -> str:
-    """Builds the C++ BaseSimulation wrapper (host/demo5.cpp) into the test
-    driver tests/cpp/demo5_driver, linked against libnbs_sph.so."""
-    host = os.path.join(HERE, "host")
-    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{os.path.join(ROOT, 'include')}",
-           f"-I{host}", os.path.join(host, "demo5.cpp"), DRIVER + ".cpp", f"-L{HERE}", "-lnbs_sph",
-           f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../nbodysimulation_experiment_amd", "-o", DRIVER]
+def _build_host_program(main_source: str, out: str, rpath_rel: str, verbose: bool) -> str:
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-ffp-contract=off", "-Wall", "-Wextra",
+           f"-I{os.path.join(ROOT, 'include')}", f"-I{HOST}", *HOST_SOURCES, main_source, f"-L{HERE}", "-lnbs_sph",
+           f"-Wl,-rpath,{HERE}", f"-Wl,-rpath,$ORIGIN/{rpath_rel}", "-o", out]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError(f"host driver build failed ({r.returncode})")
-    return DRIVER
+        raise RuntimeError(f"host build of {os.path.basename(out)} failed ({r.returncode})")
+    return out
+
+
+def build_host_driver(verbose: bool = False) -> str:
+    """Builds the C++ BaseSimulation wrapper (host/demo5.cpp) into the test
+    driver tests/cpp/demo5_driver, linked against libnbs_sph.so."""
+    return _build_host_program(DRIVER + ".cpp", DRIVER, "../../nbodysimulation_experiment_amd", verbose)
+
+
+def build_benchmark(verbose: bool = False) -> str:
+    """Builds the scenario table and built-in benchmark front end
+    (host/nbs_benchmark.cpp) into nbodysimulation_experiment_amd/nbs_benchmark."""
+    return _build_host_program(os.path.join(HOST, "nbs_benchmark.cpp"), BENCHMARK, ".", verbose)
 
 
 if __name__ == "__main__":
     print(build_library(verbose=True))
     print(build_host_driver(verbose=True))
+    print(build_benchmark(verbose=True))

@@ -64,6 +64,7 @@ def lib():
         L.orc_add_particles.restype = i64
         L.orc_add_particles.argtypes = [vp, i32, vp, vp, i64]
         L.orc_set_gravity.argtypes = [vp, f, f]
+        L.orc_set_spacing.argtypes = [vp, f]
         L.orc_set_external_force.argtypes = [vp, f, f]
         L.orc_step.argtypes = [vp, f, i32]
         L.orc_add_emitter.argtypes = [vp, i32, f, f, f, f, f, f, f, f, ctypes.c_uint64]
@@ -180,6 +181,10 @@ class OracleSim:
     def add_emitter(self, scene, pos, direction, radius, speed, rate, duration, seed=0):
         lib().orc_add_emitter(self.h, scene, float(pos[0]), float(pos[1]), float(direction[0]), float(direction[1]),
                               float(radius), float(speed), float(rate), float(duration), int(seed))
+
+    def set_spacing(self, spacing):
+        """SPHParameters::particleSpacing used by the emitters (demo4.cpp:258)."""
+        lib().orc_set_spacing(self.h, float(spacing))
 
     def set_gravity(self, gx, gy):
         lib().orc_set_gravity(self.h, float(gx), float(gy))

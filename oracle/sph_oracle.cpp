@@ -44,6 +44,7 @@ struct SimBase {
   virtual void setGravity(float gx, float gy) = 0;
   virtual void setExternalForce(float fx, float fy) = 0;
   virtual void addEmitter(const Emitter &e) = 0;
+  virtual void setSpacing(float s) = 0;
   virtual void step(float dt, int nsteps) = 0;
   virtual int64_t count() const = 0;
   virtual int64_t totalCells() const = 0;
@@ -216,6 +217,7 @@ struct GSSim : SimBase {
   std::vector<Emitter> emitters;
   float spacing;
 
+  void setSpacing(float s) override { spacing = s; }
   GSSim(const ParamsF &pf, int threads) : spacing(pf.h * 0.5f) {
     w.p = Params<T>::from(pf);
     if (threads > 0) pool.reset(new Pool(threads));
@@ -473,6 +475,7 @@ struct JacobiSim : SimBase {
   std::vector<Emitter> emitters;
   float spacing;
 
+  void setSpacing(float s) override { spacing = s; }
   JacobiSim(const ParamsF &pf, int threads) : spacing(pf.h * 0.5f) {
     w.p = Params<T>::from(pf);
     if (threads > 1) pool.reset(new Pool(threads));
@@ -723,6 +726,7 @@ void orc_set_gravity(void *h, float gx, float gy) { ((SimBase *)h)->setGravity(g
 void orc_set_external_force(void *h, float fx, float fy) { ((SimBase *)h)->setExternalForce(fx, fy); }
 void orc_step(void *h, float dt, int nsteps) { ((SimBase *)h)->step(dt, nsteps); }
 // spacing: SPHParameters::particleSpacing (demo4.cpp:258); the constructor uses h/2.
+void orc_set_spacing(void *h, float spacing) { ((SimBase *)h)->setSpacing(spacing); }
 void orc_add_emitter(void *h, int scene, float px, float py, float dx, float dy, float radius, float speed,
                      float rate, float duration, uint64_t seed) {
   ((SimBase *)h)->addEmitter(Emitter{scene, px, py, dx, dy, radius, speed, rate, duration, 0.0f, 0.0f, true, seed, 0});

@@ -1,0 +1,122 @@
+"""The reference's eight scenarios on the GPU against the oracle, and the
+built-in benchmark harness (app.cpp:88-161, 215-275) end to end.
+
+Each scenario is loaded twice: through the C++ loader (host/scenarios.cpp, via
+``nbs_benchmark --dump-state``, which drives Demo5::ParticleSimulation the way
+DemoApplication::LoadScenario does) and, for the oracle, from the Python table
+(whose bit-for-bit agreement with the C++ table is a CPU test).  One benchmark
+iteration of 64 frames at dt = 1/60 must give the oracle's Jacobi-order FP32
+state bit for bit, emitted particles included.
+"""
+from __future__ import annotations
+
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from nbodysimulation_experiment_amd import scenarios as sc
+from nbodysimulation_experiment_amd.scenes import volume_positions
+
+from .helpers import bits
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "nbodysimulation_experiment_amd", "nbs_benchmark")
+OUT = os.path.join(ROOT, "gpurun_out")
+FRAMES = 64
+
+
+def oracle_for_scenario(s, seed=0):
+    from oracle.oracle import JACOBI, OracleSim
+    c = sc.CONSTANTS
+    o = OracleSim(sc.scenario_params(s), 32, JACOBI, 16)
+    scene = o.add_scene(float(c["boundary_width"]), float(c["boundary_height"]), float(c["grid_cell_size"]))
+    spacing = s.params["particle_spacing"]
+    o.set_spacing(spacing)
+    o.set_gravity(*map(float, s.gravity))
+    for b in s.bodies:
+        r = sc.resolve_body(b)
+        if r["kind"] == "plane":
+            o.add_plane(scene, r["normal"][0], r["normal"][1], r["distance"])
+        elif r["kind"] == "circle":
+            o.add_circle(scene, r["center"][0], r["center"][1], r["radius"])
+        else:
+            o.add_polygon(scene, np.array(r["verts"], np.float32))
+    for v in s.volumes:
+        nx, ny = sc.volume_counts(v, spacing)
+        pos = volume_positions(v.position, nx, ny, float(spacing), float(s.params["kernel_height"]), seed)
+        o.add_particles(scene, pos, np.tile(np.array(v.force, np.float32), (pos.shape[0], 1)))
+    for e in s.emitters:
+        o.add_emitter(scene, e.position, e.direction, e.radius, e.speed, e.rate, e.duration, seed=seed)
+    return o
+
+
+def read_dump(path):
+    raw = open(path, "rb").read()
+    n = int(np.frombuffer(raw[:8], np.uint64)[0])
+    d = np.frombuffer(raw[8:], np.float32).reshape(n, 12)
+    return {"x": d[:, 0:2], "prev": d[:, 2:4], "v": d[:, 6:8], "density": d[:, 8:10], "pressure": d[:, 10:12]}
+
+
+def assert_same(got, want, what):
+    for k in ("x", "prev", "v", "density", "pressure"):
+        g, w = bits(got[k]), bits(np.asarray(want[k], np.float32))
+        assert g.shape == w.shape, f"{what} {k}: shape {g.shape} vs {w.shape}"
+        if not np.array_equal(g, w):
+            i = int(np.argwhere(g != w)[0][0])
+            raise AssertionError(f"{what} {k}: {int((g != w).any(axis=1).sum())} particles differ; first {i}: "
+                                 f"gpu {got[k][i].tolist()} oracle {np.asarray(want[k])[i].tolist()}")
+
+
+@pytest.mark.parametrize("index", range(8))
+def test_scenario_cpp_loader_matches_oracle(index, tmp_path):
+    s = sc.SCENARIOS[index]
+    print(f"scenario {index} {s.name}: {FRAMES} frames", flush=True)
+    path = str(tmp_path / "state.bin")
+    r = subprocess.run([EXE, "--dump-state", path, "--scenario", str(index), "--frames", str(FRAMES), "--seed", "7"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = read_dump(path)
+    o = oracle_for_scenario(s, seed=7)
+    o.step(float(sc.CONSTANTS["delta_time"]), FRAMES)
+    want = o.state()
+    assert got["x"].shape[0] == o.n > 0
+    assert_same(got, want, f"{s.name}:")
+
+
+@pytest.mark.parametrize("name", ["Blob x 2", "Fun"])
+def test_scenario_python_loader_matches_oracle(name):
+    s = sc.SCENARIOS[[t.name for t in sc.SCENARIOS].index(name)]
+    with sc.make_simulation(s) as sim:
+        sc.load_scenario(sim, s, seed=3)
+        sim.Update(float(sc.CONSTANTS["delta_time"]), FRAMES)
+        got = sim.get_state()
+    o = oracle_for_scenario(s, seed=3)
+    o.step(float(sc.CONSTANTS["delta_time"]), FRAMES)
+    assert_same(got, o.state(), f"{name}:")
+
+
+def test_benchmark_harness_cpp_and_python():
+    r = subprocess.run([EXE, "--scenario", "0", "--frames", "16", "--iterations", "3", "--json"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    b = json.loads(r.stdout.strip().splitlines()[-1])
+    assert b["scenario"] == "Dambreak" and b["frames"] == 16 and b["iterations"] == 3
+    assert b["particles"] == [5300, 5300, 5300]
+    mn, mx, av = b["simulation_ms"]
+    assert 0.0 < mn <= av <= mx
+    for p in ("viscosityForces", "updateGrid", "densityAndPressure", "deltaPositions"):
+        lo, hi, mean = b["phases_ms"][p]
+        assert 0.0 < lo <= mean <= hi, p
+    with sc.make_simulation(4) as sim:
+        res = sc.run_benchmark(sim, 4, frames=16, iterations=2)
+    assert res["scenario"] == "Liquid"
+    lo, hi, _ = res["particles"]
+    assert 0 < lo < hi  # the emitter adds particles during each iteration
+    os.makedirs(OUT, exist_ok=True)
+    with open(os.path.join(OUT, "scenario_benchmark_sample.json"), "w") as f:
+        json.dump({"cpp": b, "python": res}, f, indent=1)
