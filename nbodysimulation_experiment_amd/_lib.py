@@ -1,4 +1,5 @@
-"""ctypes binding of libnbs_sph.so (the C ABI declared in include/nbs_sph.h).
+"""ctypes binding of libnbs_sph.so (the C ABI declared in include/nbs_sph.h and
+include/nbs_surrogate.h).
 
 The library is the product: there is no CPU fallback.  Loading fails loudly
 when the shared library has not been built.
@@ -13,6 +14,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libnbs_sph.so")
 HEADER_PATH = os.path.join(ROOT, "include", "nbs_sph.h")
+HEADER_PATHS = (HEADER_PATH, os.path.join(ROOT, "include", "nbs_surrogate.h"))
+SURROGATE_WEIGHT_COUNT = 9474
+SURROGATE_PARAM_COUNT = 9482
+RELAX_PLAIN = 0
+RELAX_SURROGATE = 1
 
 c_float, c_int32, c_uint32, c_uint64, c_double = (ctypes.c_float, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint64,
                                                   ctypes.c_double)
@@ -62,6 +68,12 @@ class NbsTuning(ctypes.Structure):
     _fields_ = [("autotune", c_int32), ("retune_interval", c_uint32), ("tile_cap", c_int32 * 3)]
 
 
+class NbsSurrogateReport(ctypes.Structure):
+    _fields_ = [(n, c_uint64) for n in ("surrogate_steps", "plain_steps", "check_steps", "fallbacks",
+                                        "train_steps")] + \
+               [(n, c_double) for n in ("last_density_error", "max_density_error", "mean_density_error")]
+
+
 class NbsError(RuntimeError):
     def __init__(self, code: int, func: str, message: str):
         self.code = code
@@ -109,12 +121,23 @@ SIGNATURES = {
     "nbs_is_multithreading_supported": [vp, P(c_int32)],
     "nbs_selftest_math": [vp, vp, c_uint64, vp],
     "nbs_bench_copy": [c_uint64, c_int32, P(c_double)],
+    "nbs_surrogate_init": [vp, c_uint64],
+    "nbs_surrogate_set_params": [vp, vp, c_uint64],
+    "nbs_surrogate_get_params": [vp, vp, c_uint64],
+    "nbs_surrogate_calibrate": [vp, c_float],
+    "nbs_surrogate_train": [vp, c_float, c_uint32, c_float, vp],
+    "nbs_surrogate_set_mode": [vp, c_int32, c_uint32, c_float],
+    "nbs_surrogate_get_report": [vp, P(NbsSurrogateReport)],
+    "nbs_surrogate_frame": [vp, c_float, vp, vp, vp, vp, vp, vp, vp, vp],
+    "nbs_selftest_mfma": [vp, vp, vp],
 }
 
 
-def declared_symbols(header: str = HEADER_PATH) -> list[str]:
-    """Every function the C header declares."""
-    text = open(header).read()
+def declared_symbols(headers=HEADER_PATHS) -> list[str]:
+    """Every function the C headers declare."""
+    if isinstance(headers, str):
+        headers = (headers,)
+    text = "".join(open(h).read() for h in headers)
     return sorted(set(re.findall(r"^\s*(?:nbs_status|const char \*)\s*\**\s*(nbs_\w+)\s*\(", text, re.M)))
 
 

@@ -496,6 +496,7 @@ __global__ __launch_bounds__(kBlock) void k_delta(Buffers b, DevParams p, DevSce
     delta_sum(b.startNext, sc, cx, cy, tx, b.pres, TileIndex(plan), p, xi, pi, accx, accy);
   else
     delta_sum(b.startNext, sc, cx, cy, b.x1, b.pres, SlotIndex(), p, xi, pi, accx, accy);
+  if (b.dxOut) b.dxOut[i] = make_float2(accx, accy);
   float x = accx + xi.x;                                  // demo4.cpp:253
   float y = accy + xi.y;
   collide(b, p, sc, x, y);                                // demo4.cpp:411-444

@@ -70,6 +70,8 @@ struct Buffers {
   int nScenes;
   const DevBody *bodies;
   const float2 *verts;
+  float2 *dxOut;       // if set, the relaxation kernel also writes each particle's
+                       // relaxation displacement (before collisions) here
 };
 
 // Launchers (nbs_kernels.hip).  All kernels are enqueued on `stream`.

@@ -12,9 +12,14 @@
 #include <vector>
 
 #include "../../include/nbs_sph.h"
+#include "../../include/nbs_surrogate.h"
 #include "nbs_kernels.hpp"
+#include "nbs_surrogate.hpp"
 
 namespace nbs {
+
+static_assert(NBS_SURROGATE_PARAM_COUNT == kNumParams, "surrogate parameter count");
+static_assert(NBS_SURROGATE_WEIGHT_COUNT == kNumWeights, "surrogate weight count");
 
 static thread_local std::string g_lastError;
 
@@ -389,7 +394,7 @@ class Simulation {
         continue;
       }
       const bool graphable = (options_ & NBS_OPT_HIP_GRAPH) && !timing_ && emitters_.empty() && lists_ &&
-                             nBinned_ == n_;
+                             nBinned_ == n_ && !surrogateMode();
       if (graphable) {
         // Replay pairs of steps (the cell tables and the slot-id arrays
         // alternate between two buffers every step).
@@ -412,8 +417,11 @@ class Simulation {
         ensureEvents(slots + 1);
         ev = (int)slots++;
       }
-      enqueueStep(dt, ev);
+      const int relax = surrogateMode() ? nextRelax() : RELAX_PLAIN;
+      if (relax != RELAX_PLAIN) ensureSurInference();
+      enqueueStep(dt, ev, relax);
       swapAfterStep();
+      if (relax == RELAX_CHECK) finishCheck();
       ++stepsSinceTune_;
       ++s;
     }
@@ -439,6 +447,153 @@ class Simulation {
   nbs_tuning tuning() const { return tuning_; }
 
   hipStream_t stream() const { return stream_; }
+
+  // ---- learned relaxation ---------------------------------------------------
+  void surInit(uint64_t seed) {
+    if (!sur_) sur_.reset(new SurrogateState());
+    SurrogateState &S = *sur_;
+    S.params.alloc(kNumParams);
+    S.adamM.alloc(kNumWeights);
+    S.adamV.alloc(kNumWeights);
+    S.grad.alloc(kNumWeights);
+    S.w1b.alloc(kHid * kFeat);
+    S.w2b.alloc(kHid * kHid);
+    S.w3b.alloc(kHid * kHid);
+    std::vector<float> h(kNumParams, 0.0f);
+    uint64_t draw = 0;
+    auto fill = [&](int off, int count, float bound) {
+      for (int k = 0; k < count; ++k) h[off + k] = bound * splitmix_uniform(seed, draw++);
+    };
+    fill(kOffW1, kHid * kFeat, std::sqrt(6.0f / 11.0f));      // 11 features in use
+    fill(kOffW2, kHid * kHid, std::sqrt(6.0f / kHid) / 8.0f);  // inputs are sums over pairs
+    fill(kOffW3, kHid * kHid, std::sqrt(6.0f / kHid));
+    fill(kOffW4, 2 * kHid, 0.1f * std::sqrt(6.0f / kHid));
+    const float norm[8] = {1.0f, 0.0f, 1.0f, 0.0f, 1.0f, 1.0f, 0.0f, 0.0f};
+    std::memcpy(&h[kOffNorm], norm, sizeof(norm));
+    uploadParams(h.data());
+    std::vector<float> zero(kNumWeights, 0.0f);
+    NBS_HIP(hipMemcpyAsync(S.adamM.p, zero.data(), kNumWeights * 4, hipMemcpyHostToDevice, stream_));
+    NBS_HIP(hipMemcpyAsync(S.adamV.p, zero.data(), kNumWeights * 4, hipMemcpyHostToDevice, stream_));
+    NBS_HIP(hipStreamSynchronize(stream_));
+    S.adamT = 0;
+    S.normSet = false;
+    S.rep = nbs_surrogate_report{};
+    S.errSum = 0.0;
+  }
+  void surSetParams(const float *h, uint64_t count) {
+    if (!h || count != (uint64_t)kNumParams)
+      throw Error(NBS_ERR_INVALID_ARGUMENT, "the parameter vector has NBS_SURROGATE_PARAM_COUNT floats");
+    if (!sur_) surInit(0);
+    uploadParams(h);
+    sur_->normSet = true;
+  }
+  void surGetParams(float *h, uint64_t count) {
+    requireSur();
+    if (!h || count != (uint64_t)kNumParams)
+      throw Error(NBS_ERR_INVALID_ARGUMENT, "the parameter vector has NBS_SURROGATE_PARAM_COUNT floats");
+    NBS_HIP(hipMemcpyAsync(h, sur_->params.p, kNumParams * 4, hipMemcpyDeviceToHost, stream_));
+    NBS_HIP(hipStreamSynchronize(stream_));
+  }
+  void surSetMode(int mode, uint32_t interval, float maxErr) {
+    if (mode != NBS_RELAX_PLAIN && mode != NBS_RELAX_SURROGATE) throw Error(NBS_ERR_INVALID_ARGUMENT, "unknown mode");
+    if (mode == NBS_RELAX_SURROGATE) {
+      requireSur();
+      if (interval == 0) throw Error(NBS_ERR_INVALID_ARGUMENT, "check_interval must be at least 1");
+      if (!(maxErr >= 0.0f) || !std::isfinite(maxErr))
+        throw Error(NBS_ERR_INVALID_ARGUMENT, "max_density_error must be finite and non-negative");
+    }
+    if (!sur_) return;
+    SurrogateState &S = *sur_;
+    S.mode = mode;
+    S.interval = interval;
+    S.maxErr = maxErr;
+    S.sinceCheck = 0;
+    S.plainLeft = 0;
+    S.rep.surrogate_steps = S.rep.plain_steps = S.rep.check_steps = S.rep.fallbacks = 0;
+    S.rep.last_density_error = S.rep.max_density_error = S.rep.mean_density_error = 0.0;
+    S.errSum = 0.0;
+    destroyGraph();
+  }
+  nbs_surrogate_report surReport() const {
+    if (!sur_) return nbs_surrogate_report{};
+    nbs_surrogate_report r = sur_->rep;
+    r.mean_density_error = r.check_steps ? sur_->errSum / (double)r.check_steps : 0.0;
+    return r;
+  }
+  void surCalibrate(float dt) {
+    requireSur();
+    std::vector<float> x, prev, dens, target;
+    runCapturedStep(dt, RELAX_CALIBRATE, 0, &x, &prev, &dens, &target, nullptr);
+    const uint64_t n = n_;
+    double sr = 0, sr2 = 0, sn = 0, sn2 = 0, sv2 = 0, so2 = 0;
+    const double invDt = (double)(1.0f / dt);
+    for (uint64_t i = 0; i < n; ++i) {
+      const double r = dens[2 * i], rn = dens[2 * i + 1];
+      sr += r, sr2 += r * r, sn += rn, sn2 += rn * rn;
+      const double vx = ((double)x[2 * i] - prev[2 * i]) * invDt, vy = ((double)x[2 * i + 1] - prev[2 * i + 1]) * invDt;
+      sv2 += vx * vx + vy * vy;
+      so2 += (double)target[2 * i] * target[2 * i] + (double)target[2 * i + 1] * target[2 * i + 1];
+    }
+    const double N = (double)std::max<uint64_t>(n, 1);
+    const double mr = sr / N, mn = sn / N;
+    const double vr = std::max(sr2 / N - mr * mr, 0.0), vn = std::max(sn2 / N - mn * mn, 0.0);
+    float norm[8];
+    norm[0] = (float)(1.0 / std::max(std::sqrt(sv2 / (2.0 * N)), 1e-6));
+    norm[1] = (float)mr;
+    norm[2] = (float)(1.0 / std::max(std::sqrt(vr), 1e-6));
+    norm[3] = (float)mn;
+    norm[4] = (float)(1.0 / std::max(std::sqrt(vn), 1e-6));
+    norm[5] = (float)std::max(std::sqrt(so2 / (2.0 * N)), 1e-12);
+    norm[6] = norm[7] = 0.0f;
+    NBS_HIP(hipMemcpyAsync(sur_->params.p + kOffNorm, norm, sizeof(norm), hipMemcpyHostToDevice, stream_));
+    NBS_HIP(hipStreamSynchronize(stream_));
+    sur_->normSet = true;
+  }
+  void surTrain(float dt, uint32_t steps, float lr, float *losses) {
+    requireSur();
+    if (!(lr > 0.0f) || !std::isfinite(lr)) throw Error(NBS_ERR_INVALID_ARGUMENT, "learning_rate must be positive");
+    if (!(dt > 0.0f) || !std::isfinite(dt)) throw Error(NBS_ERR_INVALID_ARGUMENT, "dt must be positive and finite");
+    if (!sur_->normSet) surCalibrate(dt);
+    SurrogateState &S = *sur_;
+    if (S.loss.n < std::max<uint32_t>(steps, 1)) S.loss.alloc(std::max<uint32_t>(steps, 1));
+    S.lr = lr;
+    syncBodies();
+    for (uint32_t s = 0; s < steps; ++s) {
+      if (!emitters_.empty()) runEmitters(dt);
+      if (n_ == 0) continue;
+      ensureSurTraining();
+      enqueueStep(dt, -1, RELAX_TRAIN, (int)s);
+      swapAfterStep();
+      ++stepsSinceTune_;
+    }
+    NBS_HIP(hipGetLastError());
+    NBS_HIP(hipStreamSynchronize(stream_));
+    if (losses && steps) NBS_HIP(hipMemcpy(losses, S.loss.p, steps * 4, hipMemcpyDeviceToHost));
+    statsPending_ = true;
+    destroyGraph();
+  }
+  void surFrame(float dt, float *grad, float *loss, float *out, float *x, float *prev, float *dens, float *target,
+                int32_t *scene) {
+    requireSur();
+    std::vector<float> hx, hp, hd, ht, ho;
+    std::vector<uint32_t> keys;
+    runCapturedStep(dt, RELAX_FRAME, 0, x ? &hx : nullptr, prev ? &hp : nullptr, dens ? &hd : nullptr,
+                    target ? &ht : nullptr, scene ? &keys : nullptr);
+    const uint64_t n = n_;
+    if (grad) NBS_HIP(hipMemcpy(grad, sur_->grad.p, kNumWeights * 4, hipMemcpyDeviceToHost));
+    if (loss) NBS_HIP(hipMemcpy(loss, sur_->loss.p, 4, hipMemcpyDeviceToHost));
+    if (out && n) NBS_HIP(hipMemcpy(out, sur_->out.p, n * 8, hipMemcpyDeviceToHost));
+    if (x) std::memcpy(x, hx.data(), n * 8);
+    if (prev) std::memcpy(prev, hp.data(), n * 8);
+    if (dens) std::memcpy(dens, hd.data(), n * 8);
+    if (target) std::memcpy(target, ht.data(), n * 8);
+    if (scene) {
+      std::vector<uint32_t> bases;
+      for (const HostScene &sc : scenes_) bases.push_back(sc.cellBase);
+      for (uint64_t i = 0; i < n; ++i)
+        scene[i] = (int32_t)(std::upper_bound(bases.begin(), bases.end(), keys[i]) - bases.begin() - 1);
+    }
+  }
 
  private:
   // FP32 derivations in the reference's order.
@@ -607,8 +762,9 @@ class Simulation {
     destroyGraph();
   }
 
-  // One step on stream_ (ev >= 0: record timing events for step slot ev).
-  void enqueueStep(float dt, int ev) {
+  // One step on stream_ (ev >= 0: record timing events for step slot ev);
+  // relax selects how the relaxation phase runs (Relax).
+  void enqueueStep(float dt, int ev, int relax = RELAX_PLAIN, int lossIndex = 0) {
     const DevParams p = devParams(dt);
     const DevScene s0 = hostScenes_[0];
     const int n = (int)n_;
@@ -621,7 +777,11 @@ class Simulation {
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 2], stream_));
     launch_density(buf_, p, s0, n, cap[1], stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 3], stream_));
-    launch_delta(buf_, p, s0, n, cap[2], stream_);
+    if (relax == RELAX_PLAIN) {
+      launch_delta(buf_, p, s0, n, cap[2], stream_);
+    } else {
+      enqueueLearnedRelaxation(p, s0, n, relax, lossIndex);
+    }
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 4], stream_));
   }
 
@@ -636,7 +796,7 @@ class Simulation {
   uint32_t tuneVariants() const { return tuned_ ? 3u : (uint32_t)kNumTileCaps; }
   uint32_t tuneSteps() const { return kTuneReps * tuneVariants(); }
   bool wantTune() const {
-    if (!tuning_.autotune || n_ < kTuneMinParticles || !lists_ || nBinned_ != n_) return false;
+    if (!tuning_.autotune || n_ < kTuneMinParticles || !lists_ || nBinned_ != n_ || surrogateMode()) return false;
     return !tuned_ || (tuning_.retune_interval > 0 && stepsSinceTune_ >= tuning_.retune_interval);
   }
   static int capIndex(int cap) {
@@ -798,6 +958,214 @@ class Simulation {
     }
   }
 
+  // ---- learned relaxation (internals) ----------------------------------------
+  enum Relax { RELAX_PLAIN = 0, RELAX_SURROGATE = 1, RELAX_CHECK = 2, RELAX_TRAIN = 3, RELAX_FRAME = 4,
+               RELAX_CALIBRATE = 5 };
+
+  struct SurrogateState {
+    DevBuf<float> params, adamM, adamV, grad, out, act, cnt, dH, dG, dM, dO, part, lossPart, loss;
+    DevBuf<uint16_t> w1b, w2b, w3b;
+    DevBuf<float2> target, shadowX, shadowV, densA, densB, presScratch;
+    DevBuf<uint32_t> candScratch;
+    DevBuf<double> errPartials, errOut;
+    size_t inferCap = 0, trainCap = 0;
+    bool normSet = false;
+    int mode = NBS_RELAX_PLAIN;
+    uint32_t interval = 8;
+    float maxErr = 0.01f;
+    float lr = 1.0e-3f;
+    uint64_t adamT = 0;
+    uint64_t sinceCheck = 0;
+    uint32_t plainLeft = 0;
+    nbs_surrogate_report rep{};
+    double errSum = 0.0;
+  };
+
+  bool surrogateMode() const { return sur_ && sur_->mode == NBS_RELAX_SURROGATE; }
+  void requireSur() const {
+    if (!sur_) throw Error(NBS_ERR_INVALID_ARGUMENT, "the surrogate is not initialised (nbs_surrogate_init)");
+  }
+  void uploadParams(const float *h) {
+    SurrogateState &S = *sur_;
+    NBS_HIP(hipMemcpyAsync(S.params.p, h, kNumParams * 4, hipMemcpyHostToDevice, stream_));
+    launch_bf16_weights(S.params.p, S.w1b.p, S.w2b.p, S.w3b.p, stream_);
+    NBS_HIP(hipGetLastError());
+    NBS_HIP(hipStreamSynchronize(stream_));
+  }
+  // Buffers of surrogate inference and the density check, sized to cap_.
+  void ensureSurInference() {
+    SurrogateState &S = *sur_;
+    if (S.inferCap >= cap_) return;
+    const size_t c = cap_;
+    S.shadowX.alloc(c);
+    S.shadowV.alloc(c);
+    S.densA.alloc(c);
+    S.densB.alloc(c);
+    S.presScratch.alloc(c);
+    S.candScratch.alloc(2 * ((c + kBlock - 1) / kBlock));
+    S.errPartials.alloc(2 * std::max<size_t>((c + 2047) / 2048, 1));
+    S.errOut.alloc(2);
+    S.inferCap = c;
+  }
+  void ensureSurTraining() {
+    SurrogateState &S = *sur_;
+    ensureSurInference();
+    if (S.trainCap >= cap_) return;
+    const size_t c = cap_;
+    S.out.alloc(2 * c);
+    S.act.alloc((size_t)kActStride * c);
+    S.cnt.alloc(c);
+    S.dH.alloc(kHid * c);
+    S.dG.alloc(kHid * c);
+    S.dM.alloc(kHid * c);
+    S.dO.alloc(2 * c);
+    S.target.alloc(c);
+    S.part.alloc(surrogate_part_floats((int)c));
+    S.lossPart.alloc(surrogate_head_blocks((int)c));
+    if (S.loss.n == 0) S.loss.alloc(1);
+    S.trainCap = c;
+  }
+  SurrogateBufs surBufs(bool training, int lossIndex) const {
+    const SurrogateState &S = *sur_;
+    SurrogateBufs m{};
+    m.params = S.params.p;
+    m.w1b = S.w1b.p;
+    m.w2b = S.w2b.p;
+    m.w3b = S.w3b.p;
+    if (training) {
+      m.out = S.out.p;
+      m.act = S.act.p;
+      m.cnt = S.cnt.p;
+      m.dH = S.dH.p;
+      m.dG = S.dG.p;
+      m.dM = S.dM.p;
+      m.dO = S.dO.p;
+      m.target = S.target.p;
+      m.part = S.part.p;
+      m.grad = S.grad.p;
+      m.lossPart = S.lossPart.p;
+      m.loss = S.loss.p;
+      m.lossIndex = lossIndex;
+    }
+    return m;
+  }
+  // The relaxation phase of a step in the learned modes.
+  void enqueueLearnedRelaxation(const DevParams &p, const DevScene &s0, int n, int relax, int lossIndex) {
+    SurrogateState &S = *sur_;
+    const int *cap = tuning_.tile_cap;
+    switch (relax) {
+      case RELAX_SURROGATE:
+        launch_surrogate_forward(buf_, p, s0, n, surBufs(false, 0), 1, stream_);
+        break;
+      case RELAX_CHECK: {
+        // plain result into the shadow state, the network's into the state
+        Buffers bp = buf_;
+        bp.x0 = S.shadowX.p;
+        bp.v = S.shadowV.p;
+        launch_delta(bp, p, s0, n, cap[2], stream_);
+        launch_surrogate_forward(buf_, p, s0, n, surBufs(false, 0), 1, stream_);
+        // density of both results on this step's grid
+        Buffers bd = buf_;
+        bd.pres = S.presScratch.p;
+        bd.candStats = S.candScratch.p;
+        bd.x1 = buf_.x0;
+        bd.dens = S.densA.p;
+        launch_density(bd, p, s0, n, cap[1], stream_);
+        bd.x1 = S.shadowX.p;
+        bd.dens = S.densB.p;
+        launch_density(bd, p, s0, n, cap[1], stream_);
+        launch_density_error(S.densA.p, S.densB.p, n, S.errPartials.p, S.errOut.p, stream_);
+        break;
+      }
+      case RELAX_TRAIN:
+      case RELAX_FRAME:
+      case RELAX_CALIBRATE: {
+        Buffers bt = buf_;
+        bt.dxOut = S.target.p;
+        launch_delta(bt, p, s0, n, cap[2], stream_);
+        if (relax == RELAX_CALIBRATE) break;
+        const SurrogateBufs m = surBufs(true, lossIndex);
+        launch_surrogate_forward(buf_, p, s0, n, m, 2, stream_);
+        launch_surrogate_backward(buf_, p, s0, n, m, stream_);
+        if (relax == RELAX_TRAIN) {
+          launch_adam(S.params.p, S.grad.p, S.adamM.p, S.adamV.p, kNumWeights, S.lr, 0.9f, 0.999f, 1.0e-8f,
+                      (int)++S.adamT, stream_);
+          launch_bf16_weights(S.params.p, S.w1b.p, S.w2b.p, S.w3b.p, stream_);
+          ++S.rep.train_steps;
+        }
+        break;
+      }
+      default:
+        launch_delta(buf_, p, s0, n, cap[2], stream_);
+    }
+  }
+  // Relaxation of the next step in surrogate mode.
+  int nextRelax() {
+    SurrogateState &S = *sur_;
+    if (S.plainLeft > 0) {
+      --S.plainLeft;
+      ++S.rep.plain_steps;
+      return RELAX_PLAIN;
+    }
+    if (++S.sinceCheck >= S.interval) {
+      S.sinceCheck = 0;
+      return RELAX_CHECK;
+    }
+    ++S.rep.surrogate_steps;
+    return RELAX_SURROGATE;
+  }
+  void finishCheck() {
+    SurrogateState &S = *sur_;
+    double e[2];
+    NBS_HIP(hipMemcpyAsync(e, S.errOut.p, sizeof(e), hipMemcpyDeviceToHost, stream_));
+    NBS_HIP(hipStreamSynchronize(stream_));
+    const double err = e[1] != 0.0 ? e[0] / std::fabs(e[1]) : (e[0] != 0.0 ? INFINITY : 0.0);
+    ++S.rep.check_steps;
+    S.rep.last_density_error = err;
+    S.rep.max_density_error = std::max(S.rep.max_density_error, err);
+    S.errSum += err;
+    if (!(err <= (double)S.maxErr)) {
+      NBS_HIP(hipMemcpyAsync(buf_.x0, S.shadowX.p, n_ * sizeof(float2), hipMemcpyDeviceToDevice, stream_));
+      NBS_HIP(hipMemcpyAsync(buf_.v, S.shadowV.p, n_ * sizeof(float2), hipMemcpyDeviceToDevice, stream_));
+      ++S.rep.fallbacks;
+      ++S.rep.plain_steps;
+      S.plainLeft = S.interval;
+    } else {
+      ++S.rep.surrogate_steps;
+    }
+  }
+  // One plain step that also captures the frame (relax: RELAX_CALIBRATE or
+  // RELAX_FRAME); copies the requested slot-order arrays to the host.
+  void runCapturedStep(float dt, int relax, int lossIndex, std::vector<float> *x, std::vector<float> *prev,
+                       std::vector<float> *dens, std::vector<float> *target, std::vector<uint32_t> *keys) {
+    if (!(dt > 0.0f) || !std::isfinite(dt)) throw Error(NBS_ERR_INVALID_ARGUMENT, "dt must be positive and finite");
+    syncBodies();
+    if (!emitters_.empty()) runEmitters(dt);
+    if (n_ == 0) throw Error(NBS_ERR_INVALID_ARGUMENT, "no particles");
+    ensureSurTraining();
+    enqueueStep(dt, -1, relax, lossIndex);
+    swapAfterStep();
+    ++stepsSinceTune_;
+    NBS_HIP(hipGetLastError());
+    const size_t n = n_;
+    auto get2 = [&](std::vector<float> *v, const float2 *src) {
+      if (!v) return;
+      v->resize(2 * n);
+      NBS_HIP(hipMemcpyAsync(v->data(), src, n * sizeof(float2), hipMemcpyDeviceToHost, stream_));
+    };
+    get2(x, buf_.x1);
+    get2(prev, buf_.prev);
+    get2(dens, buf_.dens);
+    get2(target, sur_->target.p);
+    if (keys) {
+      keys->resize(n);
+      NBS_HIP(hipMemcpyAsync(keys->data(), buf_.keys, n * 4, hipMemcpyDeviceToHost, stream_));
+    }
+    NBS_HIP(hipStreamSynchronize(stream_));
+    statsPending_ = true;
+    destroyGraph();
+  }
+
   struct Storage {
     DevBuf<float2> x0, xp, x1, prev, v, dens, pres, acc;
     DevBuf<uint32_t> id, id2, keys, keyp, fill, tmpSlot, tmpId, scn, count, startCur, startNext, blockSum;
@@ -846,6 +1214,8 @@ class Simulation {
   hipGraphExec_t graphExec_ = nullptr;
   DevParams graphParams_{};
   uint64_t graphN_ = 0;
+
+  std::unique_ptr<SurrogateState> sur_;
 };
 
 }  // namespace nbs
@@ -1154,6 +1524,66 @@ nbs_status nbs_bench_copy(uint64_t bytes, int32_t reps, double *gbps) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *gbps = 2.0 * (double)(bytes / 16 * 16) * reps / ((double)ms * 1e-3) / 1e9;
+  });
+}
+
+// ---- learned relaxation (include/nbs_surrogate.h) ---------------------------
+nbs_status nbs_surrogate_init(nbs_sim *sim, uint64_t seed) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->surInit(seed); });
+}
+
+nbs_status nbs_surrogate_set_params(nbs_sim *sim, const float *params, uint64_t count) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->surSetParams(params, count); });
+}
+
+nbs_status nbs_surrogate_get_params(nbs_sim *sim, float *params, uint64_t count) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->surGetParams(params, count); });
+}
+
+nbs_status nbs_surrogate_calibrate(nbs_sim *sim, float dt) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->surCalibrate(dt); });
+}
+
+nbs_status nbs_surrogate_train(nbs_sim *sim, float dt, uint32_t steps, float learning_rate, float *losses) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->surTrain(dt, steps, learning_rate, losses); });
+}
+
+nbs_status nbs_surrogate_set_mode(nbs_sim *sim, int32_t mode, uint32_t check_interval, float max_density_error) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->surSetMode(mode, check_interval, max_density_error); });
+}
+
+nbs_status nbs_surrogate_get_report(nbs_sim *sim, nbs_surrogate_report *report) {
+  NBS_CHECK_SIM(sim);
+  if (!report) return NBS_ERR_INVALID_ARGUMENT;
+  return guarded([&] { *report = sim->impl->surReport(); });
+}
+
+nbs_status nbs_surrogate_frame(nbs_sim *sim, float dt, float *grad, float *loss, float *out, float *x, float *prev,
+                               float *density, float *target, int32_t *scene) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->surFrame(dt, grad, loss, out, x, prev, density, target, scene); });
+}
+
+nbs_status nbs_selftest_mfma(const float *a, const float *b, float *c) {
+  if (!a || !b || !c) return NBS_ERR_INVALID_ARGUMENT;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return NBS_ERR_NO_DEVICE;
+  return guarded([&] {
+    nbs::DevBuf<float> da, db, dc;
+    da.alloc(32 * 8);
+    db.alloc(8 * 32);
+    dc.alloc(32 * 32);
+    NBS_HIP(hipMemcpy(da.p, a, 32 * 8 * 4, hipMemcpyHostToDevice));
+    NBS_HIP(hipMemcpy(db.p, b, 8 * 32 * 4, hipMemcpyHostToDevice));
+    nbs::launch_mfma_probe(da.p, db.p, dc.p, nullptr);
+    NBS_HIP(hipGetLastError());
+    NBS_HIP(hipMemcpy(c, dc.p, 32 * 32 * 4, hipMemcpyDeviceToHost));
   });
 }
 

@@ -9,7 +9,7 @@ reference's interface reads the same here.  Every call goes to the HIP library
 Extensions beyond the reference's interface: several independent scenes per
 object (``AddScene`` and the ``scene`` keyword), bulk ``AddParticles``,
 ``get_state``/``set_state``, ``get_grid``, FP64 ``diagnostics``, per-kernel
-timing and HIP-graph replay.
+timing, HIP-graph replay, and the learned relaxation (``surrogate_*``).
 """
 from __future__ import annotations
 
@@ -18,10 +18,10 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import NbsDiagnostics, NbsDomain, NbsParams, NbsStats, NbsTuning, call
+from ._lib import NbsDiagnostics, NbsDomain, NbsParams, NbsStats, NbsSurrogateReport, NbsTuning, call
 from .scenes import SPHParams
 
-__all__ = ["ParticleSimulation", "NbsError", "params_to_c"]
+__all__ = ["ParticleSimulation", "NbsError", "params_to_c", "selftest_math", "selftest_mfma", "bench_copy"]
 
 NbsError = _lib.NbsError
 
@@ -294,6 +294,54 @@ class ParticleSimulation:
         call("nbs_get_kernel_times", self._h, _ptr(avg), _ptr(cnt))
         return {k: (float(a), int(c)) for k, a, c in zip(_lib.KERNEL_NAMES, avg, cnt)}
 
+    # -- learned relaxation (include/nbs_surrogate.h) ---------------------------
+    def surrogate_init(self, seed: int = 0):
+        call("nbs_surrogate_init", self._h, int(seed))
+
+    def surrogate_params(self) -> np.ndarray:
+        p = np.zeros(_lib.SURROGATE_PARAM_COUNT, np.float32)
+        call("nbs_surrogate_get_params", self._h, _ptr(p), p.size)
+        return p
+
+    def surrogate_set_params(self, params):
+        p = np.ascontiguousarray(params, np.float32).reshape(-1)
+        call("nbs_surrogate_set_params", self._h, _ptr(p), p.size)
+
+    def surrogate_calibrate(self, dt: float):
+        call("nbs_surrogate_calibrate", self._h, float(dt))
+
+    def surrogate_train(self, dt: float, steps: int, learning_rate: float = 1e-3) -> np.ndarray:
+        losses = np.zeros(max(int(steps), 1), np.float32)
+        call("nbs_surrogate_train", self._h, float(dt), int(steps), float(learning_rate), _ptr(losses))
+        return losses[:int(steps)]
+
+    def surrogate_set_mode(self, surrogate: bool, check_interval: int = 8, max_density_error: float = 0.01):
+        mode = _lib.RELAX_SURROGATE if surrogate else _lib.RELAX_PLAIN
+        call("nbs_surrogate_set_mode", self._h, mode, int(check_interval), float(max_density_error))
+
+    def surrogate_report(self) -> dict:
+        r = NbsSurrogateReport()
+        call("nbs_surrogate_get_report", self._h, ctypes.byref(r))
+        return {n: (getattr(r, n) if t is ctypes.c_double else int(getattr(r, n))) for n, t in r._fields_}
+
+    def surrogate_frame(self, dt: float) -> dict:
+        """One plain step; the frame the network saw (slot order) and the
+        gradient of the loss on it, without a weight update."""
+        n0 = self.GetParticleCount()
+        cap = n0 + 4096  # room for particles an emitter adds during the step
+        grad = np.zeros(_lib.SURROGATE_WEIGHT_COUNT, np.float32)
+        loss = np.zeros(1, np.float32)
+        arrs = {k: np.zeros((cap, 2), np.float32) for k in ("out", "x", "prev", "density", "target")}
+        scene = np.zeros(cap, np.int32)
+        call("nbs_surrogate_frame", self._h, float(dt), _ptr(grad), _ptr(loss), _ptr(arrs["out"]), _ptr(arrs["x"]),
+             _ptr(arrs["prev"]), _ptr(arrs["density"]), _ptr(arrs["target"]), _ptr(scene))
+        n = self.GetParticleCount()
+        if n > cap:
+            raise RuntimeError("an emitter added more particles than the frame buffers hold")
+        res = {k: v[:n].copy() for k, v in arrs.items()}
+        res.update(grad=grad, loss=float(loss[0]), scene=scene[:n].copy())
+        return res
+
 
 def selftest_math(a, b) -> np.ndarray:
     a = np.ascontiguousarray(a, np.float32)
@@ -301,6 +349,15 @@ def selftest_math(a, b) -> np.ndarray:
     out = np.zeros((a.size, 6), np.float32)
     call("nbs_selftest_math", _ptr(a), _ptr(b), a.size, _ptr(out))
     return out
+
+
+def selftest_mfma(a, b) -> np.ndarray:
+    """C = A (32 x 8) B (8 x 32) with BF16-rounded operands by one MFMA."""
+    a = np.ascontiguousarray(a, np.float32).reshape(32, 8)
+    b = np.ascontiguousarray(b, np.float32).reshape(8, 32)
+    c = np.zeros((32, 32), np.float32)
+    call("nbs_selftest_mfma", _ptr(a), _ptr(b), _ptr(c))
+    return c
 
 
 def bench_copy(nbytes: int = 1 << 30, reps: int = 20) -> float:
