@@ -11,8 +11,13 @@ roofline: the dominant kernel's algorithmic bytes per launch divided by its
 average duration, measured with HIP events on the library's stream over the
 timed steps.  cpu_baseline: the reference's multi-threaded step restated in
 C++ (oracle/, Gauss-Seidel order, pooled phases), timed on this host on a
-bounded sample that starts from the GPU state in the middle of the timed
-window.
+bounded sample that starts from the GPU state at the end of the timed window.
+
+surrogate (1M dam break): the learned relaxation is trained on the GPU on
+plain frames of 200 seeded 16,384-particle dam breaks (one batched
+simulation, --train-steps Adam steps after --train-warm plain steps), then
+the same K steps are timed with the network relaxing every step and the
+plain density check every 8th step (fallback above 1% density error).
 """
 from __future__ import annotations
 
@@ -38,6 +43,8 @@ DT = 0.005
 #   delta:             R x, key, pressure, prev (28) + W x, v (16)             = 44
 KERNEL_BYTES = {"viscosity_predict": 36, "grid": 64, "density": 28, "delta": 44}
 STEP_ALG_BYTES = 172  # reference phases, one SoA FP32 pass each
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
+HID, FEAT = 64, 16
 
 
 def log(msg):
@@ -75,6 +82,32 @@ def cpu_baseline(state, specs, threads, sample_steps, warm_steps=1):
     return n * sample_steps / t, t
 
 
+def train_surrogate(args):
+    """Trains the learned relaxation on plain frames of the seeded training
+    scenes; returns the parameter vector and a summary."""
+    from nbodysimulation_experiment_amd import scenes
+    from nbodysimulation_experiment_amd.scenes import SPHParams
+    from nbodysimulation_experiment_amd.sim import ParticleSimulation
+    specs = scenes.training_scenes(count=args.train_scenes)
+    sim = ParticleSimulation.from_scenes(specs, SPHParams())
+    sim.Update(DT, args.train_warm)
+    sim.surrogate_init(0)
+    sim.surrogate_calibrate(DT)
+    t0 = time.perf_counter()
+    losses, rel = sim.surrogate_train(DT, args.train_steps, args.lr, relative=True)
+    t = time.perf_counter() - t0
+    params = sim.surrogate_params()
+    n = sim.GetParticleCount()
+    sim.close()
+    k = max(1, len(losses) // 10)
+    return params, {"scenes": len(specs), "particles": n, "warm_steps": args.train_warm, "steps": args.train_steps,
+                    "learning_rate": args.lr, "seconds": round(t, 3),
+                    "particle_steps_per_s": round(n * args.train_steps / t, 1),
+                    "relative_error_first": float(rel[:k].mean()), "relative_error_last": float(rel[-k:].mean()),
+                    "relative_error_definition": "sum|o - t/os|^2 / sum|t/os|^2 per training frame (first / last "
+                                                 "10% of the steps)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +120,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="also run the library's copy kernel on MiB of data (3 times) for counter calibration")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-relaxation leg")
+    ap.add_argument("--train-scenes", type=int, default=200)
+    ap.add_argument("--train-warm", type=int, default=100)
+    ap.add_argument("--train-steps", type=int, default=500)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--check-interval", type=int, default=8)
+    ap.add_argument("--max-density-error", type=float, default=0.01)
     ap.add_argument("--tile-cap", default="auto",
                     help="LDS tile capacities of the viscosity, density and relaxation kernels, e.g. 2048,2048,1024; "
                          "'auto' lets the library's autotuner choose")
@@ -135,6 +175,7 @@ def main():
     tuning = sim.get_tuning()
     stats = sim.GetStats()
     diag = sim.diagnostics()
+    end_state = sim.get_state() if not args.no_cpu_baseline else None
     sim.close()
     log(f"{value:.4g} particle-steps/s, {1e3 * elapsed / args.steps:.4f} ms/step; kernels {kt}")
 
@@ -156,19 +197,62 @@ def main():
                     "copy_ceiling_gbs": round(copy_gbs, 1),
                     "step_alg_bytes_frac": round(value * STEP_ALG_BYTES / 1e9 / HBM_PEAK_GBS, 5)}
 
+    surrogate = None
+    if args.workload == "dam-break-1m" and not args.no_surrogate and not args.graph:
+        log(f"training the learned relaxation: {args.train_scenes} scenes, {args.train_steps} steps")
+        params, train = train_surrogate(args)
+        log(f"training: {train}")
+        warm = make_sim()
+        warm.surrogate_set_params(params)
+        warm.surrogate_set_mode(True, args.check_interval, args.max_density_error)
+        warm.Update(DT, args.warmup)
+        warm.close()
+        ssim = make_sim()
+        ssim.surrogate_set_params(params)
+        ssim.surrogate_set_mode(True, args.check_interval, args.max_density_error)
+        ssim.set_timing(True)
+        ssim.ResetStats()
+        log(f"timing {args.steps} surrogate steps")
+        t0 = time.perf_counter()
+        ssim.Update(DT, args.steps)
+        s_elapsed = time.perf_counter() - t0
+        s_kt = ssim.kernel_times()
+        rep = ssim.surrogate_report()
+        sdiag = ssim.diagnostics()
+        ssim.close()
+        s_value = n * args.steps / s_elapsed
+        relax_ms = s_kt["delta"][0]
+        pairs = rep["last_pair_count"]
+        flops = 2.0 * HID * FEAT * pairs + 2.0 * 2 * HID * HID * n
+        surrogate = {
+            "value": round(s_value, 1), "unit": "particle-steps/s", "ms_per_step": round(1e3 * s_elapsed / args.steps, 5),
+            "speedup_vs_plain": round(s_value / value, 4),
+            "density_error": {"definition": "sum|rho(surrogate) - rho(plain)| / sum rho(plain) on check steps",
+                              "max": rep["max_density_error"], "mean": rep["mean_density_error"],
+                              "limit": args.max_density_error, "check_interval": args.check_interval,
+                              "checks": rep["check_steps"], "fallbacks": rep["fallbacks"]},
+            "steps_network": rep["surrogate_steps"], "steps_plain": rep["plain_steps"],
+            "relaxation_ms": round(relax_ms, 5),
+            "mfma": {"flops_per_step": flops, "pairs_per_step": pairs,
+                     "achieved_tflops": round(flops / (relax_ms * 1e-3) / 1e12, 3),
+                     "peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+                     "frac": round(flops / (relax_ms * 1e-3) / 1e12 / MFMA_BF16_PEAK_TFLOPS, 6),
+                     "note": "pair layer 2*64*16 flops per pair + head 2*2*64*64 per particle, over the relaxation "
+                             "phase's average time (check steps add a plain relaxation and two density passes)"},
+            "training": train,
+            "kernels_ms": {k: round(v[0], 5) for k, v in s_kt.items()},
+            "diagnostics_fp64": {k: sdiag[k] for k in ("mean_density", "max_density", "non_finite_count")},
+        }
+        log(f"surrogate {s_value:.4g} particle-steps/s ({s_value / value:.3f}x plain); report {rep}")
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1)
-        mid = args.steps // 2
-        mid_sim = make_sim()
-        mid_sim.Update(DT, mid)
-        state = mid_sim.get_state()
-        mid_sim.close()
-        log(f"cpu baseline: {threads} threads, {args.cpu_steps} steps from step {mid}")
-        cv, ct = cpu_baseline(state, specs, threads, args.cpu_steps)
+        log(f"cpu baseline: {threads} threads, {args.cpu_steps} steps from step {args.steps}")
+        cv, ct = cpu_baseline(end_state, specs, threads, args.cpu_steps)
         cpu = {"value": round(cv, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the same scene from the GPU state at step {mid} (the middle of "
-                         f"the timed window; +1 untimed step to rebuild the neighbour lists), {ct:.1f} s",
+               "sample": f"{args.cpu_steps} steps of the same scene from the GPU state at the end of the timed "
+                         f"window (step {args.steps}; +1 untimed step to rebuild the neighbour lists), {ct:.1f} s",
                "gpu_over_cpu": round(value / cv, 1)}
         log(f"cpu baseline {cv:.4g} particle-steps/s")
 
@@ -178,7 +262,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": desc, "particles": n, "dt": DT, "method": "plain", "hip_graph": args.graph,
                    "tuning": tuning},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "surrogate": surrogate,
         "stats": {k: v for k, v in stats.items() if k != "time"},
         "diagnostics_fp64": {k: diag[k] for k in ("mean_density", "max_density", "mean_abs_density_error",
                                                   "kinetic_energy", "non_finite_count")},

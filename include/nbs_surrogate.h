@@ -52,6 +52,7 @@ typedef struct nbs_surrogate_report {
   double last_density_error;
   double max_density_error;
   double mean_density_error;  /* over the checks since the last nbs_surrogate_set_mode */
+  uint64_t last_pair_count;   /* in-range pairs the last network evaluation processed */
 } nbs_surrogate_report;
 
 /* Seeded initial weights (He-uniform, zero biases) and neutral normalisation;
@@ -65,10 +66,13 @@ nbs_status nbs_surrogate_get_params(nbs_sim *sim, float *params, uint64_t count)
  * relaxation displacement. */
 nbs_status nbs_surrogate_calibrate(nbs_sim *sim, float dt);
 /* Runs `steps` plain steps; after each, one Adam step (learning_rate,
- * betas 0.9 / 0.999, eps 1e-8) on that frame.  losses[steps] (may be NULL)
- * receives each frame's loss before its update.  Calibrates first if the
- * normalisation was never set. */
-nbs_status nbs_surrogate_train(nbs_sim *sim, float dt, uint32_t steps, float learning_rate, float *losses);
+ * betas 0.9 / 0.999, eps 1e-8) on that frame.  losses[steps] and
+ * relative_losses[steps] (either may be NULL) receive each frame's loss
+ * before its update: the mean squared error, and the squared error relative
+ * to the squared normalised target, sum |o - t/os|^2 / sum |t/os|^2.
+ * Calibrates first if the normalisation was never set. */
+nbs_status nbs_surrogate_train(nbs_sim *sim, float dt, uint32_t steps, float learning_rate, float *losses,
+                               float *relative_losses);
 nbs_status nbs_surrogate_set_mode(nbs_sim *sim, int32_t mode, uint32_t check_interval, float max_density_error);
 nbs_status nbs_surrogate_get_report(nbs_sim *sim, nbs_surrogate_report *report);
 /* Runs one plain step and, without updating the weights, returns the frame

@@ -71,7 +71,8 @@ class NbsTuning(ctypes.Structure):
 class NbsSurrogateReport(ctypes.Structure):
     _fields_ = [(n, c_uint64) for n in ("surrogate_steps", "plain_steps", "check_steps", "fallbacks",
                                         "train_steps")] + \
-               [(n, c_double) for n in ("last_density_error", "max_density_error", "mean_density_error")]
+               [(n, c_double) for n in ("last_density_error", "max_density_error", "mean_density_error")] + \
+               [("last_pair_count", c_uint64)]
 
 
 class NbsError(RuntimeError):
@@ -125,7 +126,7 @@ SIGNATURES = {
     "nbs_surrogate_set_params": [vp, vp, c_uint64],
     "nbs_surrogate_get_params": [vp, vp, c_uint64],
     "nbs_surrogate_calibrate": [vp, c_float],
-    "nbs_surrogate_train": [vp, c_float, c_uint32, c_float, vp],
+    "nbs_surrogate_train": [vp, c_float, c_uint32, c_float, vp, vp],
     "nbs_surrogate_set_mode": [vp, c_int32, c_uint32, c_float],
     "nbs_surrogate_get_report": [vp, P(NbsSurrogateReport)],
     "nbs_surrogate_frame": [vp, c_float, vp, vp, vp, vp, vp, vp, vp, vp],

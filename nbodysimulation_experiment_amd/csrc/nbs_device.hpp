@@ -226,4 +226,58 @@ __device__ __forceinline__ RowRange row_range(const uint32_t *start, const DevSc
   return {start[row + xlo], start[row + xhi + 1]};
 }
 
+// LDS tile of a workgroup (positions only: every candidate's position is read
+// by the cheap pass, the other per-particle data only for in-kernel pairs, from
+// the slot arrays).  The workgroup's particles occupy consecutive
+// sorted slots, so their keys span [kA, kB] and the candidates of all of them
+// lie in three slot ranges, one per row offset dy: the cells with keys in
+// [kA + dy*nx - 1, kB + dy*nx + 1] (clamped to the scene).  Each range is
+// staged at tile index off[d]; a slot j of range d sits at off[d] + j - rowStart[d].
+struct TilePlan {
+  uint32_t rowStart[3];
+  uint32_t off[4];  // off[3] = staged particles
+  int use;          // 1 when the ranges fit the tile
+};
+
+template <int CAP>
+__device__ __forceinline__ void plan_tile(const uint32_t *start, const DevScene &sc, uint32_t kA, uint32_t kB,
+                                          TilePlan &t) {
+  const int64_t first = sc.cellBase, last = (int64_t)sc.cellBase + (int64_t)sc.nx * sc.ny - 1;
+  uint32_t total = 0;
+  for (int d = 0; d < 3; ++d) {
+    const int64_t lo = max((int64_t)kA + (int64_t)(d - 1) * sc.nx - 1, first);
+    const int64_t hi = min((int64_t)kB + (int64_t)(d - 1) * sc.nx + 1, last);
+    uint32_t a = 0, e = 0;
+    if (lo <= hi) {
+      a = start[lo];
+      e = start[hi + 1];
+    }
+    t.rowStart[d] = a;
+    t.off[d] = total;
+    total += e - a;
+  }
+  t.off[3] = total;
+  t.use = (CAP > 0 && total <= (uint32_t)CAP) ? 1 : 0;
+}
+
+// Cooperative, coalesced copy of the planned ranges of src into the tile.
+__device__ __forceinline__ void stage(const TilePlan &t, const float2 *__restrict__ src, float2 *dst) {
+  for (uint32_t q = threadIdx.x; q < t.off[3]; q += blockDim.x) {
+    const int d = (q >= t.off[1] ? 1 : 0) + (q >= t.off[2] ? 1 : 0);
+    dst[q] = src[t.rowStart[d] + (q - t.off[d])];
+  }
+}
+
+// Index of slot j of row range d in the neighbour data source.
+struct TileIndex {
+  uint32_t rowStart[3], off[3];
+  __device__ __forceinline__ explicit TileIndex(const TilePlan &t) {
+    for (int d = 0; d < 3; ++d) rowStart[d] = t.rowStart[d], off[d] = t.off[d];
+  }
+  __device__ __forceinline__ uint32_t operator()(int d, uint32_t j) const { return off[d] + (j - rowStart[d]); }
+};
+struct SlotIndex {
+  __device__ __forceinline__ uint32_t operator()(int, uint32_t j) const { return j; }
+};
+
 }  // namespace nbs

@@ -514,10 +514,17 @@ class Simulation {
     S.errSum = 0.0;
     destroyGraph();
   }
-  nbs_surrogate_report surReport() const {
+  nbs_surrogate_report surReport() {
     if (!sur_) return nbs_surrogate_report{};
     nbs_surrogate_report r = sur_->rep;
     r.mean_density_error = r.check_steps ? sur_->errSum / (double)r.check_steps : 0.0;
+    r.last_pair_count = 0;
+    if (sur_->pairWaves > 0 && sur_->pairPart.p) {
+      std::vector<uint32_t> c((size_t)sur_->pairWaves);
+      NBS_HIP(hipMemcpyAsync(c.data(), sur_->pairPart.p, c.size() * 4, hipMemcpyDeviceToHost, stream_));
+      NBS_HIP(hipStreamSynchronize(stream_));
+      for (uint32_t v : c) r.last_pair_count += v;
+    }
     return r;
   }
   void surCalibrate(float dt) {
@@ -549,13 +556,13 @@ class Simulation {
     NBS_HIP(hipStreamSynchronize(stream_));
     sur_->normSet = true;
   }
-  void surTrain(float dt, uint32_t steps, float lr, float *losses) {
+  void surTrain(float dt, uint32_t steps, float lr, float *losses, float *relLosses) {
     requireSur();
     if (!(lr > 0.0f) || !std::isfinite(lr)) throw Error(NBS_ERR_INVALID_ARGUMENT, "learning_rate must be positive");
     if (!(dt > 0.0f) || !std::isfinite(dt)) throw Error(NBS_ERR_INVALID_ARGUMENT, "dt must be positive and finite");
     if (!sur_->normSet) surCalibrate(dt);
     SurrogateState &S = *sur_;
-    if (S.loss.n < std::max<uint32_t>(steps, 1)) S.loss.alloc(std::max<uint32_t>(steps, 1));
+    if (S.loss.n < 2 * (size_t)std::max<uint32_t>(steps, 1)) S.loss.alloc(2 * (size_t)std::max<uint32_t>(steps, 1));
     S.lr = lr;
     syncBodies();
     for (uint32_t s = 0; s < steps; ++s) {
@@ -568,7 +575,14 @@ class Simulation {
     }
     NBS_HIP(hipGetLastError());
     NBS_HIP(hipStreamSynchronize(stream_));
-    if (losses && steps) NBS_HIP(hipMemcpy(losses, S.loss.p, steps * 4, hipMemcpyDeviceToHost));
+    if ((losses || relLosses) && steps) {
+      std::vector<float> l(2 * (size_t)steps);
+      NBS_HIP(hipMemcpy(l.data(), S.loss.p, l.size() * 4, hipMemcpyDeviceToHost));
+      for (uint32_t k = 0; k < steps; ++k) {
+        if (losses) losses[k] = l[2 * k];
+        if (relLosses) relLosses[k] = l[2 * k + 1];
+      }
+    }
     statsPending_ = true;
     destroyGraph();
   }
@@ -966,9 +980,10 @@ class Simulation {
     DevBuf<float> params, adamM, adamV, grad, out, act, cnt, dH, dG, dM, dO, part, lossPart, loss;
     DevBuf<uint16_t> w1b, w2b, w3b;
     DevBuf<float2> target, shadowX, shadowV, densA, densB, presScratch;
-    DevBuf<uint32_t> candScratch;
+    DevBuf<uint32_t> candScratch, pairPart;
     DevBuf<double> errPartials, errOut;
     size_t inferCap = 0, trainCap = 0;
+    int pairWaves = 0;  // pair counts written by the last network evaluation
     bool normSet = false;
     int mode = NBS_RELAX_PLAIN;
     uint32_t interval = 8;
@@ -1005,6 +1020,7 @@ class Simulation {
     S.candScratch.alloc(2 * ((c + kBlock - 1) / kBlock));
     S.errPartials.alloc(2 * std::max<size_t>((c + 2047) / 2048, 1));
     S.errOut.alloc(2);
+    S.pairPart.alloc(surrogate_forward_waves((int)c));
     S.inferCap = c;
   }
   void ensureSurTraining() {
@@ -1021,8 +1037,8 @@ class Simulation {
     S.dO.alloc(2 * c);
     S.target.alloc(c);
     S.part.alloc(surrogate_part_floats((int)c));
-    S.lossPart.alloc(surrogate_head_blocks((int)c));
-    if (S.loss.n == 0) S.loss.alloc(1);
+    S.lossPart.alloc(2 * (size_t)surrogate_head_blocks((int)c));
+    if (S.loss.n < 2) S.loss.alloc(2);
     S.trainCap = c;
   }
   SurrogateBufs surBufs(bool training, int lossIndex) const {
@@ -1032,6 +1048,7 @@ class Simulation {
     m.w1b = S.w1b.p;
     m.w2b = S.w2b.p;
     m.w3b = S.w3b.p;
+    m.pairPart = S.pairPart.p;
     if (training) {
       m.out = S.out.p;
       m.act = S.act.p;
@@ -1052,6 +1069,7 @@ class Simulation {
   // The relaxation phase of a step in the learned modes.
   void enqueueLearnedRelaxation(const DevParams &p, const DevScene &s0, int n, int relax, int lossIndex) {
     SurrogateState &S = *sur_;
+    if (relax != RELAX_CALIBRATE) S.pairWaves = surrogate_forward_waves(n);
     const int *cap = tuning_.tile_cap;
     switch (relax) {
       case RELAX_SURROGATE:
@@ -1548,9 +1566,10 @@ nbs_status nbs_surrogate_calibrate(nbs_sim *sim, float dt) {
   return guarded([&] { sim->impl->surCalibrate(dt); });
 }
 
-nbs_status nbs_surrogate_train(nbs_sim *sim, float dt, uint32_t steps, float learning_rate, float *losses) {
+nbs_status nbs_surrogate_train(nbs_sim *sim, float dt, uint32_t steps, float learning_rate, float *losses,
+                               float *relative_losses) {
   NBS_CHECK_SIM(sim);
-  return guarded([&] { sim->impl->surTrain(dt, steps, learning_rate, losses); });
+  return guarded([&] { sim->impl->surTrain(dt, steps, learning_rate, losses, relative_losses); });
 }
 
 nbs_status nbs_surrogate_set_mode(nbs_sim *sim, int32_t mode, uint32_t check_interval, float max_density_error) {

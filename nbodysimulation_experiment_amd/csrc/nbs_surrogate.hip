@@ -24,17 +24,17 @@ namespace nbs {
 typedef short bf16x4 __attribute__((__vector_size__(4 * sizeof(short))));
 typedef float floatx16 __attribute__((__vector_size__(16 * sizeof(float))));
 
-// FP32 -> BF16, round to nearest even (NaN stays NaN).
-__device__ __forceinline__ uint16_t f2bf(float x) {
-  uint32_t u = __float_as_uint(x);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32).
 __device__ __forceinline__ uint32_t pack2(float lo, float hi) {
-  return (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+  bf16x2 v;
+  v[0] = (__bf16)lo;
+  v[1] = (__bf16)hi;
+  return __builtin_bit_cast(uint32_t, v);
 }
+__device__ __forceinline__ uint16_t f2bf(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
 __device__ __forceinline__ bf16x4 words_to_bf4(uint32_t w0, uint32_t w1) {
   bf16x4 v;
   v[0] = (short)(w0 & 0xffffu);
@@ -69,6 +69,11 @@ __device__ __forceinline__ int c_row(int r, int g) { return 8 * (r >> 2) + 4 * g
 constexpr int kWaveRows = 64;    // particles per wave in the pair kernels
 constexpr int kHbStride = 68;    // BF16 per row of the head operand buffer (padded)
 constexpr int kGStride = kHid + 1;
+constexpr int kHbBytes = kSurRows * kHbStride * 2;
+constexpr int kHeadBytes = kHbBytes + kSurRows * kGStride * 4;
+constexpr int kTileCap = 512;    // staged candidates (positions, previous positions, densities)
+constexpr int kStageBytes = 3 * kTileCap * 8;
+constexpr int kPoolBytes = kHeadBytes > kStageBytes ? kHeadBytes : kStageBytes;
 
 // Per-particle inputs of the pair layer.
 struct PartInfo {
@@ -99,41 +104,95 @@ __device__ __forceinline__ PartInfo part_info(const Buffers &b, const DevParams 
 }
 
 // A lane's walk over its particle's candidates: the rows dy = -1..1 of the
-// 3x3 cells, each a contiguous slot range (demo4.cpp:188-203 order).
+// 3x3 cells, each a contiguous slot range (demo4.cpp:188-203 order).  tb maps
+// a slot of the current row to its index in the neighbour source.
 struct Cursor {
-  uint32_t j, jend;
+  uint32_t j, jend, tb;
   int d;
   bool done;
 };
 
-__device__ __forceinline__ bool cursor_next_row(Cursor &cu, const uint32_t *start, const PartInfo &q) {
+// Neighbour data of a wave's candidates: an LDS copy of the three candidate
+// slot ranges of the wave's 64 particles (the layout of TilePlan) when they
+// fit, else the slot arrays.
+
+struct PairSrc {
+  const float2 *x, *pv, *d;
+  uint32_t rowStart0, rowStart1, rowStart2, off0, off1, off2;
+  bool tiled;
+  // slot j of candidate row r sits at j + base(r)
+  __device__ __forceinline__ uint32_t base(int r) const {
+    if (!tiled) return 0u;
+    return r == 0 ? off0 - rowStart0 : (r == 1 ? off1 - rowStart1 : off2 - rowStart2);
+  }
+};
+
+__device__ __forceinline__ bool cursor_next_row(Cursor &cu, const uint32_t *start, const PartInfo &q,
+                                                const PairSrc &src) {
   while (cu.d < 3) {
     const RowRange rr = row_range(start, q.sc, q.cx, q.cy, cu.d - 1);
-    ++cu.d;
+    const int r = cu.d++;
     if (rr.j0 < rr.j1) {
       cu.j = rr.j0;
       cu.jend = rr.j1;
+      cu.tb = src.base(r);
       return true;
     }
   }
   return false;
 }
 
+// Plans the wave's tile and stages it when it fits (every lane must call it).
+template <bool MULTI>
+__device__ __forceinline__ PairSrc stage_pairs(const Buffers &b, const DevScene &s0, int base, int n,
+                                               unsigned char *pool, TilePlan &plan) {
+  if (threadIdx.x == 0) {
+    const uint32_t kA = b.keys[base], kB = b.keys[min(base + kWaveRows, n) - 1];
+    const DevScene sa = scene_of_slot<MULTI>(b, s0, kA);
+    plan_tile<kTileCap>(b.startNext, sa, kA, kB, plan);
+    if (MULTI && sa.cellBase != scene_of_slot<MULTI>(b, s0, kB).cellBase) plan.use = 0;
+  }
+  __syncthreads();
+  PairSrc src;
+  src.tiled = plan.use != 0;
+  src.rowStart0 = plan.rowStart[0], src.rowStart1 = plan.rowStart[1], src.rowStart2 = plan.rowStart[2];
+  src.off0 = plan.off[0], src.off1 = plan.off[1], src.off2 = plan.off[2];
+  if (src.tiled) {
+    float2 *tx = reinterpret_cast<float2 *>(pool);
+    float2 *tp = tx + kTileCap, *td = tp + kTileCap;
+    stage(plan, b.x1, tx);
+    stage(plan, b.prev, tp);
+    stage(plan, b.dens, td);
+    src.x = tx;
+    src.pv = tp;
+    src.d = td;
+  } else {
+    src.x = b.x1;
+    src.pv = b.prev;
+    src.d = b.dens;
+  }
+  __syncthreads();
+  return src;
+}
+
 // Advances to the lane's next pair (j != i, r^2 < h^2); false when exhausted.
-__device__ __forceinline__ bool next_pair(Cursor &cu, const Buffers &b, const DevParams &p, const PartInfo &q, int i,
-                                          uint32_t &jOut, float2 &xj, float &Rx, float &Ry, float &r2) {
+// t is the pair's index in src.
+__device__ __forceinline__ bool next_pair(Cursor &cu, const Buffers &b, const DevParams &p, const PairSrc &src,
+                                          const PartInfo &q, int i, uint32_t &t, float2 &xj, float &Rx, float &Ry,
+                                          float &r2) {
   while (!cu.done) {
-    if (cu.j >= cu.jend && !cursor_next_row(cu, b.startNext, q)) {
+    if (cu.j >= cu.jend && !cursor_next_row(cu, b.startNext, q, src)) {
       cu.done = true;
       break;
     }
     const uint32_t j = cu.j++;
-    xj = b.x1[j];
+    const uint32_t k = j + cu.tb;
+    xj = src.x[k];
     Rx = xj.x - q.xi.x;
     Ry = xj.y - q.xi.y;
     r2 = Rx * Rx + Ry * Ry;
     if (r2 < p.h2 && j != (uint32_t)i) {
-      jOut = j;
+      t = k;
       return true;
     }
   }
@@ -141,10 +200,10 @@ __device__ __forceinline__ bool next_pair(Cursor &cu, const Buffers &b, const De
 }
 
 // The pair's 16 BF16 features as 8 words (features 2k, 2k+1 in word k).
-__device__ __forceinline__ void pair_words(uint32_t w[8], const Buffers &b, const DevParams &p, const float *norm,
-                                           const PartInfo &q, uint32_t j, float2 xj, float Rx, float Ry, float r2) {
-  const float2 pj = b.prev[j];
-  const float2 dj = b.dens[j];
+__device__ __forceinline__ void pair_words(uint32_t w[8], const PairSrc &src, const DevParams &p, const float *norm,
+                                           const PartInfo &q, uint32_t t, float2 xj, float Rx, float Ry, float r2) {
+  const float2 pj = src.pv[t];
+  const float2 dj = src.d[t];
   const float vjx = (xj.x - pj.x) * p.invDt, vjy = (xj.y - pj.y) * p.invDt;
   const float r = sqrt_rn(r2);
   const float qq = 1.0f - r * p.invH;
@@ -200,21 +259,26 @@ __device__ __forceinline__ void pair_layer(bf16x4 as0, bf16x4 as1, const W1Opera
 // Forward
 // ---------------------------------------------------------------------------
 template <bool MULTI>
-__global__ __launch_bounds__(64) void k_sur_forward(Buffers b, DevParams p, DevScene s0, int n, SurrogateBufs m,
+__global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, DevScene s0, int n, SurrogateBufs m,
                                                     int mode) {
-  __shared__ uint16_t hb[kSurRows][kHbStride];    // H, then M, of one row tile as BF16 operand rows
-  __shared__ float gbuf[kSurRows][kGStride];      // G (FP32) of one row tile
+  // LDS: the staged candidates during the pair layer, then the head's operand
+  // rows (hb: H, then M, of one row tile as BF16) and G (FP32)
+  __shared__ __align__(16) unsigned char pool[kPoolBytes];
+  __shared__ TilePlan plan;
   __shared__ float shCnt[kWaveRows];
+  uint16_t(*hb)[kHbStride] = reinterpret_cast<uint16_t(*)[kHbStride]>(pool);
+  float(*gbuf)[kGStride] = reinterpret_cast<float(*)[kGStride]>(pool + kHbBytes);
   const int lane = threadIdx.x, c = lane & 31, g = lane >> 5;
   const int base = (int)blockIdx.x * kWaveRows;
   const int me = base + lane;
   const float *prm = m.params;
   const float *norm = prm + kOffNorm;
   const W1Operands w1 = load_w1(m.w1b);
+  const PairSrc src = stage_pairs<MULTI>(b, s0, base, n, pool, plan);
 
   // 1. pair layer: H_i = sum_j relu(W1 f_ij)
   PartInfo q{};
-  Cursor cu{0u, 0u, 0, true};
+  Cursor cu{0u, 0u, 0u, 0, true};
   if (me < n) {
     q = part_info<MULTI>(b, p, s0, norm, me);
     cu.done = false;
@@ -222,14 +286,14 @@ __global__ __launch_bounds__(64) void k_sur_forward(Buffers b, DevParams p, DevS
   floatx16 h00 = zero16(), h01 = zero16(), h10 = zero16(), h11 = zero16();  // tile, hidden half
   uint32_t np = 0;
   for (;;) {
-    uint32_t j = 0;
+    uint32_t t = 0;
     float2 xj = make_float2(0.0f, 0.0f);
     float Rx = 0.0f, Ry = 0.0f, r2 = 0.0f;
-    const bool have = next_pair(cu, b, p, q, me, j, xj, Rx, Ry, r2);
+    const bool have = next_pair(cu, b, p, src, q, me, t, xj, Rx, Ry, r2);
     if (__ballot(have) == 0ull) break;  // wave-uniform
     uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
     if (have) {
-      pair_words(w, b, p, norm, q, j, xj, Rx, Ry, r2);
+      pair_words(w, src, p, norm, q, t, xj, Rx, Ry, r2);
       ++np;
     }
     const PairA a = pair_operands(w);
@@ -248,6 +312,12 @@ __global__ __launch_bounds__(64) void k_sur_forward(Buffers b, DevParams p, DevS
     }
   }
   shCnt[lane] = (float)np;
+  if (m.pairPart) {
+    uint32_t tot = np;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) tot += (uint32_t)__shfl_xor((int)tot, off, 64);
+    if (lane == 0) m.pairPart[blockIdx.x] = tot;
+  }
 
   // 2. head, one 32-row tile at a time: C = A W^T with A in hb (BF16)
   auto dense = [&](const uint16_t *W, floatx16 &olo, floatx16 &ohi) {
@@ -340,63 +410,92 @@ __global__ __launch_bounds__(64) void k_sur_forward(Buffers b, DevParams p, DevS
 // ---------------------------------------------------------------------------
 // Backward
 // ---------------------------------------------------------------------------
-// Per particle (FP32): dO, dG (pre-activation), dM, dH and the squared error.
+// Per particle (FP32): dO, dG (pre-activation), dM, dH, the squared error and
+// the squared normalised target.  The loops are unrolled so the per-particle
+// vectors stay in registers; the weights are wave-uniform (scalar) loads.
 __global__ __launch_bounds__(kBlock) void k_sur_back_head(int n, SurrogateBufs m, float invN) {
-  __shared__ float red[kBlock / 64];
+  __shared__ float red[2][kBlock / 64];
   const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
-  float err = 0.0f;
+  float err = 0.0f, ref = 0.0f;
   if (i < n) {
     const float *prm = m.params;
     const float os = prm[kOffNorm + 5];
     const float *G = m.act + (size_t)i * kActStride + 2 * kHid;
     const float2 t = m.target[i];
-    const float e0 = m.out[2 * (size_t)i] - t.x / os;
-    const float e1 = m.out[2 * (size_t)i + 1] - t.y / os;
+    const float t0 = t.x / os, t1 = t.y / os;
+    const float e0 = m.out[2 * (size_t)i] - t0;
+    const float e1 = m.out[2 * (size_t)i + 1] - t1;
     err = e0 * e0 + e1 * e1;
+    ref = t0 * t0 + t1 * t1;
     const float d0 = 2.0f * invN * e0, d1 = 2.0f * invN * e1;
     m.dO[2 * (size_t)i] = d0;
     m.dO[2 * (size_t)i + 1] = d1;
-    float gpre[kHid];
+    float *dG = m.dG + (size_t)i * kHid, *dM = m.dM + (size_t)i * kHid, *dH = m.dH + (size_t)i * kHid;
+    float vin[kHid], acc[kHid];
+#pragma clang loop unroll(full)
     for (int k = 0; k < kHid; ++k) {
       const float v = prm[kOffW4 + k] * d0 + prm[kOffW4 + kHid + k] * d1;
-      gpre[k] = G[k] > 0.0f ? v : 0.0f;
-      m.dG[(size_t)i * kHid + k] = gpre[k];
+      vin[k] = G[k] > 0.0f ? v : 0.0f;
+      dG[k] = vin[k];
+      acc[k] = 0.0f;
     }
-    float dm[kHid];
+    // dM = W3^T dG, then dH = W2^T dM: row r of W adds W[r][:] * d[r]
+#pragma clang loop unroll(full)
+    for (int r = 0; r < kHid; ++r) {
+#pragma clang loop unroll(full)
+      for (int k = 0; k < kHid; ++k) acc[k] += prm[kOffW3 + r * kHid + k] * vin[r];
+    }
+#pragma clang loop unroll(full)
     for (int k = 0; k < kHid; ++k) {
-      float s = 0.0f;
-      for (int r = 0; r < kHid; ++r) s += prm[kOffW3 + r * kHid + k] * gpre[r];
-      dm[k] = s;
-      m.dM[(size_t)i * kHid + k] = s;
+      dM[k] = acc[k];
+      vin[k] = acc[k];
+      acc[k] = 0.0f;
     }
-    for (int k = 0; k < kHid; ++k) {
-      float s = 0.0f;
-      for (int r = 0; r < kHid; ++r) s += prm[kOffW2 + r * kHid + k] * dm[r];
-      m.dH[(size_t)i * kHid + k] = s;
+#pragma clang loop unroll(full)
+    for (int r = 0; r < kHid; ++r) {
+#pragma clang loop unroll(full)
+      for (int k = 0; k < kHid; ++k) acc[k] += prm[kOffW2 + r * kHid + k] * vin[r];
     }
+#pragma clang loop unroll(full)
+    for (int k = 0; k < kHid; ++k) dH[k] = acc[k];
   }
-  // squared error, summed per workgroup
-  for (int off = 32; off > 0; off >>= 1) err += __shfl_xor(err, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = err;
+  // squared error and squared target, summed per workgroup
+  for (int off = 32; off > 0; off >>= 1) {
+    err += __shfl_xor(err, off, 64);
+    ref += __shfl_xor(ref, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = err;
+    red[1][threadIdx.x >> 6] = ref;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    float s = 0.0f;
-    for (int w = 0; w < kBlock / 64; ++w) s += red[w];
-    m.lossPart[blockIdx.x] = s;
+    float s0 = 0.0f, s1 = 0.0f;
+    for (int w = 0; w < kBlock / 64; ++w) s0 += red[0][w], s1 += red[1][w];
+    m.lossPart[2 * blockIdx.x] = s0;
+    m.lossPart[2 * blockIdx.x + 1] = s1;
   }
 }
 
+// loss[0] = mean squared error, loss[1] = sum |e|^2 / sum |target/os|^2.
 __global__ __launch_bounds__(kBlock) void k_loss_final(const float *part, int nb, float invN, float *loss) {
-  __shared__ float red[kBlock];
-  float s = 0.0f;
-  for (int k = threadIdx.x; k < nb; k += kBlock) s += part[k];
-  red[threadIdx.x] = s;
+  __shared__ float red[2][kBlock];
+  float s0 = 0.0f, s1 = 0.0f;
+  for (int k = threadIdx.x; k < nb; k += kBlock) s0 += part[2 * k], s1 += part[2 * k + 1];
+  red[0][threadIdx.x] = s0;
+  red[1][threadIdx.x] = s1;
   __syncthreads();
   for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    if ((int)threadIdx.x < w) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + w];
+      red[1][threadIdx.x] += red[1][threadIdx.x + w];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) *loss = red[0] * invN;
+  if (threadIdx.x == 0) {
+    loss[0] = red[0][0] * invN;
+    loss[1] = red[1][0] > 0.0f ? red[0][0] / red[1][0] : 0.0f;
+  }
 }
 
 // part[blockIdx][a * nb + b] = sum over the workgroup's rows of A[i][a] B[i][b];
@@ -441,9 +540,14 @@ __global__ __launch_bounds__(kBlock) void k_outer_partial(int n, const float *A,
 __global__ __launch_bounds__(kBlock) void k_sum_parts(const float *part, int nparts, int width, float *out) {
   const int e = (int)(blockIdx.x * kBlock + threadIdx.x);
   if (e >= width) return;
-  float s = 0.0f;
-  for (int q = 0; q < nparts; ++q) s += part[(size_t)q * width + e];
-  out[e] = s;
+  float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  int q = 0;
+  for (; q + 8 <= nparts; q += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] += part[(size_t)(q + u) * width + e];
+  }
+  for (; q < nparts; ++q) acc[0] += part[(size_t)q * width + e];
+  out[e] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
 // Copies a reduced [na * nb | na] block into the weight and bias slots.
@@ -463,12 +567,16 @@ __global__ __launch_bounds__(kBlock) void k_scatter_grad(const float *sum, int n
 template <bool MULTI>
 __global__ __launch_bounds__(64) void k_sur_back_pairs(Buffers b, DevParams p, DevScene s0, int n, SurrogateBufs m) {
   __shared__ uint32_t fw[kWaveRows][kFeat / 2];
+  __shared__ __align__(16) unsigned char pool[kStageBytes];
+  __shared__ TilePlan plan;
   const int lane = threadIdx.x, c = lane & 31, g = lane >> 5;
   const float *norm = m.params + kOffNorm;
   const W1Operands w1 = load_w1(m.w1b);
   floatx16 glo = zero16(), ghi = zero16();
   for (int base = (int)blockIdx.x * kWaveRows; base < n; base += (int)gridDim.x * kWaveRows) {
     const int me = base + lane;
+    __syncthreads();  // the previous group's tile is no longer read
+    const PairSrc src = stage_pairs<MULTI>(b, s0, base, n, pool, plan);
     // dH of the rows this lane's accumulator registers hold
     floatx16 d00, d01, d10, d11;  // tile, hidden half
 #pragma unroll
@@ -480,19 +588,19 @@ __global__ __launch_bounds__(64) void k_sur_back_pairs(Buffers b, DevParams p, D
       d11[k] = r1 < n ? m.dH[(size_t)r1 * kHid + 32 + c] : 0.0f;
     }
     PartInfo q{};
-    Cursor cu{0u, 0u, 0, true};
+    Cursor cu{0u, 0u, 0u, 0, true};
     if (me < n) {
       q = part_info<MULTI>(b, p, s0, norm, me);
       cu.done = false;
     }
     for (;;) {
-      uint32_t j = 0;
+      uint32_t t = 0;
       float2 xj = make_float2(0.0f, 0.0f);
       float Rx = 0.0f, Ry = 0.0f, r2 = 0.0f;
-      const bool have = next_pair(cu, b, p, q, me, j, xj, Rx, Ry, r2);
+      const bool have = next_pair(cu, b, p, src, q, me, t, xj, Rx, Ry, r2);
       if (__ballot(have) == 0ull) break;  // wave-uniform
       uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-      if (have) pair_words(w, b, p, norm, q, j, xj, Rx, Ry, r2);
+      if (have) pair_words(w, src, p, norm, q, t, xj, Rx, Ry, r2);
 #pragma unroll
       for (int k = 0; k < 8; ++k) fw[lane][k] = w[k];
       const PairA a = pair_operands(w);
@@ -505,15 +613,21 @@ __global__ __launch_bounds__(64) void k_sur_back_pairs(Buffers b, DevParams p, D
         const floatx16 &dhi = t ? d11 : d01;
 #pragma unroll
         for (int blk = 0; blk < 4; ++blk) {
-          bf16x4 bf, alo, ahi;
+          bf16x4 bf;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const uint32_t wd = fw[32 * t + 8 * blk + 4 * g + u][(c & 15) >> 1];
             const uint16_t hv = (c & 1) ? (uint16_t)(wd >> 16) : (uint16_t)(wd & 0xffffu);
             bf[u] = (short)(c < kFeat ? hv : 0);
-            alo[u] = (short)f2bf(zlo[4 * blk + u] > 0.0f ? dlo[4 * blk + u] : 0.0f);
-            ahi[u] = (short)f2bf(zhi[4 * blk + u] > 0.0f ? dhi[4 * blk + u] : 0.0f);
           }
+          float ml[4], mh[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            ml[u] = zlo[4 * blk + u] > 0.0f ? dlo[4 * blk + u] : 0.0f;
+            mh[u] = zhi[4 * blk + u] > 0.0f ? dhi[4 * blk + u] : 0.0f;
+          }
+          const bf16x4 alo = words_to_bf4(pack2(ml[0], ml[1]), pack2(ml[2], ml[3]));
+          const bf16x4 ahi = words_to_bf4(pack2(mh[0], mh[1]), pack2(mh[2], mh[3]));
           glo = mfma(alo, bf, glo);
           ghi = mfma(ahi, bf, ghi);
         }
@@ -623,6 +737,7 @@ size_t surrogate_part_floats(int n) {
 }
 
 int surrogate_head_blocks(int n) { return std::max(1, (n + kBlock - 1) / kBlock); }
+int surrogate_forward_waves(int n) { return std::max(1, (int)waves_for(n)); }
 
 void launch_surrogate_forward(const Buffers &b, const DevParams &p, const DevScene &s0, int n,
                               const SurrogateBufs &m, int mode, hipStream_t s) {
@@ -654,7 +769,7 @@ void launch_surrogate_backward(const Buffers &b, const DevParams &p, const DevSc
   const float invN = 1.0f / (float)n;
   hipLaunchKernelGGL(k_sur_back_head, dim3(hb), dim3(kBlock), 0, s, n, m, invN);
   hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(kBlock), 0, s, (const float *)m.lossPart, hb, invN,
-                     m.loss + m.lossIndex);
+                     m.loss + 2 * (size_t)m.lossIndex);
   // dW4 = sum dO G^T, db4 = sum dO
   outer_grad(n, m.dO, 2, 2, m.act + 2 * kHid, kActStride, kHid, nullptr, m.part, m.grad + kOffW4, m.grad + kOffB4, s);
   // dW3 = sum dG M^T, db3 = sum dG

@@ -62,9 +62,11 @@ struct SurrogateBufs {
   const float2 *target;      // N: plain relaxation displacement (training)
   float *part;               // partial sums of the weight gradient
   float *grad;               // kNumWeights
-  float *lossPart;           // per head workgroup
-  float *loss;               // one value per training step (loss[lossIndex])
+  float *lossPart;           // per head workgroup: squared error, squared target
+  float *loss;               // two values per training step at 2 * lossIndex:
+                             // mean squared error, relative error
   int lossIndex;
+  uint32_t *pairPart;        // per forward wave: pairs processed (may be null)
 };
 
 // Evaluates the network on the step's grid.  mode 0: outputs only (out);
@@ -73,10 +75,12 @@ struct SurrogateBufs {
 void launch_surrogate_forward(const Buffers &b, const DevParams &p, const DevScene &s0, int n,
                               const SurrogateBufs &m, int mode, hipStream_t stream);
 // Gradient of L for the frame of the last training forward into m.grad
-// (overwritten) and the loss into m.loss[m.lossIndex].
+// (overwritten) and the losses into m.loss[2 * m.lossIndex ...].
 void launch_surrogate_backward(const Buffers &b, const DevParams &p, const DevScene &s0, int n,
                                const SurrogateBufs &m, hipStream_t stream);
-// Sizes of the partial-sum buffer (floats) and of lossPart for n particles.
+// Workgroups of the forward pass (one pair count each) for n particles.
+int surrogate_forward_waves(int n);
+// Sizes of the partial-sum buffer (floats) and of lossPart (pairs) for n particles.
 size_t surrogate_part_floats(int n);
 int surrogate_head_blocks(int n);
 void launch_bf16_weights(const float *params, uint16_t *w1b, uint16_t *w2b, uint16_t *w3b, hipStream_t stream);

@@ -2,7 +2,11 @@
 
     python -m nbodysimulation_experiment_amd.profiling stats  <kernel_stats.csv>
     python -m nbodysimulation_experiment_amd.profiling counters <counter_collection.csv> [--copy-bytes N]
+    python -m nbodysimulation_experiment_amd.profiling traffic <fetch.csv> <write.csv> --copy-bytes N
+                                                              --workload W [--json out.json]
 
+``traffic`` turns one FETCH_SIZE run and one WRITE_SIZE run into bytes per
+launch of each kernel group (dispatches of every LDS-tile variant pooled).
 ``counters`` averages each counter per dispatch of each kernel.  With
 --copy-bytes, FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are converted to
 bytes with the factor that maps the copy kernel's reading onto its known
@@ -19,9 +23,19 @@ import sys
 
 SHORT = [
     ("k_visc_predict", "viscosity_predict"), ("k_scan_reduce", "scan_reduce"), ("k_scan_blocks", "scan_blocks"),
-    ("k_scan_final", "scan_final"), ("k_scatter", "scatter"), ("k_rank_reorder", "rank_reorder"),
-    ("k_density", "density"), ("k_delta", "delta"), ("k_copy", "copy"),
+    ("k_scan_final", "scan_final"), ("k_scatter_grad", "scatter_grad"), ("k_scatter", "scatter"), ("k_rank_reorder", "rank_reorder"),
+    ("k_density", "density"), ("k_delta", "delta"), ("k_copy", "copy"), ("k_sur_forward", "sur_forward"),
+    ("k_sur_back_pairs", "sur_back_pairs"), ("k_sur_back_head", "sur_back_head"),
 ]
+
+# Kernel groups of the step (timed together by the library's events).
+GROUPS = {
+    "viscosity_predict": ("viscosity_predict",),
+    "grid": ("scan_reduce", "scan_final", "scatter", "rank_reorder"),
+    "density": ("density",),
+    "delta": ("delta",),
+    "surrogate_forward": ("sur_forward",),
+}
 
 
 def short_name(name: str) -> str:
@@ -51,16 +65,74 @@ def counters(path: str) -> dict:
             for k, d in acc.items()}
 
 
+def _per_dispatch(path: str, counter: str) -> dict:
+    """{short kernel name without tile variant: [value per dispatch]} (KiB)."""
+    out = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        name = short_name(r.get("Kernel_Name", r.get("Kernel-Name", ""))).split("<")[0]
+        out[name].append(float(r["Counter_Value"]))
+    return out
+
+
+def traffic(fetch_csv: str, write_csv: str, copy_bytes: float, workload: str) -> dict:
+    """HBM bytes per launch of each kernel group, each counter scaled by the
+    factor that maps the copy kernel's reading onto its known traffic
+    (copy_bytes read and copy_bytes written per dispatch)."""
+    f = _per_dispatch(fetch_csv, "FETCH_SIZE")
+    w = _per_dispatch(write_csv, "WRITE_SIZE")
+    ff = copy_bytes / (sum(f["copy"]) / len(f["copy"]) * 1024.0)
+    wf = copy_bytes / (sum(w["copy"]) / len(w["copy"]) * 1024.0)
+    kernels, detail = {}, {}
+    for g, members in GROUPS.items():
+        tot_f = tot_w = 0.0
+        present = False
+        for k in members:
+            if k in f and k in w:
+                present = True
+                mf = sum(f[k]) / len(f[k]) * 1024.0 * ff
+                mw = sum(w[k]) / len(w[k]) * 1024.0 * wf
+                detail[k] = {"fetch_bytes": round(mf), "write_bytes": round(mw), "dispatches": len(f[k])}
+                tot_f += mf
+                tot_w += mw
+        if present:
+            kernels[g] = round(tot_f + tot_w)
+    return {"workload": workload, "unit": "bytes per launch", "fetch_factor": round(ff, 4),
+            "write_factor": round(wf, 4), "kernels": kernels, "detail": detail}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["stats", "counters"])
+    ap.add_argument("mode", choices=["stats", "counters", "traffic"])
     ap.add_argument("path")
+    ap.add_argument("path2", nargs="?")
     ap.add_argument("--copy-bytes", type=float, default=0.0)
+    ap.add_argument("--workload", default="dam-break-1m")
+    ap.add_argument("--json", default="")
     a = ap.parse_args(argv)
+    if a.mode == "traffic":
+        t = traffic(a.path, a.path2, a.copy_bytes, a.workload)
+        text = json.dumps(t, indent=1)
+        if a.json:
+            with open(a.json, "w") as fh:
+                fh.write(text + "\n")
+        print(text)
+        return
     if a.mode == "stats":
-        for r in kernel_stats(a.path):
+        rows = kernel_stats(a.path)
+        for r in rows:
             print(f"{r['kernel']:32s} calls {r['calls']:6d}  avg {r['avg_us']:9.2f} us  total {r['total_ms']:9.3f} ms"
                   f"  {r['pct']:6.2f}%")
+        # every LDS-tile variant of a single-scene kernel pooled ("<f,*>")
+        pooled = collections.defaultdict(lambda: [0, 0.0])
+        for r in rows:
+            m = re.match(r"(\w+)<f,\d+>$", r["kernel"])
+            if m:
+                pooled[m.group(1)][0] += r["calls"]
+                pooled[m.group(1)][1] += r["total_ms"]
+        for k, (calls, tot) in pooled.items():
+            print(f"{k + '<f,*>':32s} calls {calls:6d}  avg {1e3 * tot / calls:9.2f} us  total {tot:9.3f} ms  (pooled)")
         return
     c = counters(a.path)
     if a.copy_bytes and "copy" in c:

@@ -31,6 +31,7 @@ __all__ = [
     "dam_break_4k",
     "dam_break_1m",
     "batched_scenes",
+    "training_scenes",
 ]
 
 GOLDEN = np.uint64(0x9E3779B97F4A7C15)
@@ -208,4 +209,27 @@ def batched_scenes(seed: int = 2, count: int = 256, n_min: int = 2048, n_max: in
         center = (-side * 0.5 + cols * 0.25, -side * 0.5 + rows * 0.25)
         pos = volume_positions(center, cols, rows, 0.5, 1.0, seed * 1_000_003 + s, limit=n)
         scenes.append(SceneSpec(side, side, 1.0, pos, planes=wall_planes(side, side)))
+    return scenes
+
+
+def training_scenes(seed: int = 3, count: int = 200, n_side: int = 128, box_lo: float = 1.5,
+                    box_hi: float = 3.0) -> list[SceneSpec]:
+    """Frames for the learned relaxation: ``count`` dam breaks of n_side^2
+    particles (16,384 by default, 0.5h spacing) resting in the lower-left
+    corner of a box of width and height U[box_lo, box_hi] x the block side
+    each (rounded up to an even number of cells), 4 walls."""
+    u = splitmix64_uniform(seed, 2 * count)
+    block = n_side * 0.5
+    scenes = []
+    for s in range(count):
+        dims = []
+        for k in range(2):
+            side = float(math.ceil(block * (box_lo + (float(u[2 * s + k]) * 0.5 + 0.5) * (box_hi - box_lo))))
+            if int(side) % 2:
+                side += 1.0
+            dims.append(side)
+        w, h = dims
+        center = (-w * 0.5 + block * 0.5, -h * 0.5 + block * 0.5)
+        pos = volume_positions(center, n_side, n_side, 0.5, 1.0, seed * 1_000_003 + s)
+        scenes.append(SceneSpec(w, h, 1.0, pos, planes=wall_planes(w, h)))
     return scenes

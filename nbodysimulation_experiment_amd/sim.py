@@ -310,10 +310,13 @@ class ParticleSimulation:
     def surrogate_calibrate(self, dt: float):
         call("nbs_surrogate_calibrate", self._h, float(dt))
 
-    def surrogate_train(self, dt: float, steps: int, learning_rate: float = 1e-3) -> np.ndarray:
+    def surrogate_train(self, dt: float, steps: int, learning_rate: float = 1e-3, relative: bool = False):
+        """Per-step losses (mean squared error); with relative=True also the
+        relative errors sum|o - t/os|^2 / sum|t/os|^2 as a second array."""
         losses = np.zeros(max(int(steps), 1), np.float32)
-        call("nbs_surrogate_train", self._h, float(dt), int(steps), float(learning_rate), _ptr(losses))
-        return losses[:int(steps)]
+        rel = np.zeros(max(int(steps), 1), np.float32)
+        call("nbs_surrogate_train", self._h, float(dt), int(steps), float(learning_rate), _ptr(losses), _ptr(rel))
+        return (losses[:int(steps)], rel[:int(steps)]) if relative else losses[:int(steps)]
 
     def surrogate_set_mode(self, surrogate: bool, check_interval: int = 8, max_density_error: float = 0.01):
         mode = _lib.RELAX_SURROGATE if surrogate else _lib.RELAX_PLAIN
