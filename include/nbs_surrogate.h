@@ -74,6 +74,14 @@ nbs_status nbs_surrogate_calibrate(nbs_sim *sim, float dt);
 nbs_status nbs_surrogate_train(nbs_sim *sim, float dt, uint32_t steps, float learning_rate, float *losses,
                                float *relative_losses);
 nbs_status nbs_surrogate_set_mode(nbs_sim *sim, int32_t mode, uint32_t check_interval, float max_density_error);
+/* In-run training: with learning_rate > 0, every check step in surrogate mode
+ * also takes one Adam step on its own frame, whose plain relaxation the check
+ * computes anyway (0 turns it off; the normalisation must be set). */
+nbs_status nbs_surrogate_set_online_training(nbs_sim *sim, float learning_rate);
+/* Per-scene density errors of the last check step, errors[count] with count
+ * = the scene count: sum over the scene of |rho(surrogate) - rho(plain)| /
+ * sum of rho(plain) (FP64).  Zeros before the first check. */
+nbs_status nbs_surrogate_get_scene_errors(nbs_sim *sim, double *errors, uint64_t count);
 nbs_status nbs_surrogate_get_report(nbs_sim *sim, nbs_surrogate_report *report);
 /* Runs one plain step and, without updating the weights, returns the frame
  * the network saw and its gradient: grad[NBS_SURROGATE_WEIGHT_COUNT], loss,

@@ -129,6 +129,8 @@ SIGNATURES = {
     "nbs_surrogate_train": [vp, c_float, c_uint32, c_float, vp, vp],
     "nbs_surrogate_set_mode": [vp, c_int32, c_uint32, c_float],
     "nbs_surrogate_get_report": [vp, P(NbsSurrogateReport)],
+    "nbs_surrogate_set_online_training": [vp, c_float],
+    "nbs_surrogate_get_scene_errors": [vp, vp, c_uint64],
     "nbs_surrogate_frame": [vp, c_float, vp, vp, vp, vp, vp, vp, vp, vp],
     "nbs_selftest_mfma": [vp, vp, vp],
 }

@@ -418,7 +418,11 @@ class Simulation {
         ev = (int)slots++;
       }
       const int relax = surrogateMode() ? nextRelax() : RELAX_PLAIN;
-      if (relax != RELAX_PLAIN) ensureSurInference();
+      if (relax != RELAX_PLAIN) {
+        ensureSurInference();
+        ensureSceneErr();
+        if (relax == RELAX_CHECK && sur_->onlineLr > 0.0f) ensureSurTraining();
+      }
       enqueueStep(dt, ev, relax);
       swapAfterStep();
       if (relax == RELAX_CHECK) finishCheck();
@@ -513,6 +517,26 @@ class Simulation {
     S.rep.last_density_error = S.rep.max_density_error = S.rep.mean_density_error = 0.0;
     S.errSum = 0.0;
     destroyGraph();
+  }
+  void surSetOnlineTraining(float lr) {
+    requireSur();
+    if (!(lr >= 0.0f) || !std::isfinite(lr)) throw Error(NBS_ERR_INVALID_ARGUMENT, "learning_rate must be >= 0");
+    if (lr > 0.0f && !sur_->normSet)
+      throw Error(NBS_ERR_INVALID_ARGUMENT, "online training needs the normalisation (calibrate or set_params)");
+    sur_->onlineLr = lr;
+  }
+  void surSceneErrors(double *out, uint64_t count) {
+    requireSur();
+    if (!out || count != scenes_.size()) throw Error(NBS_ERR_INVALID_ARGUMENT, "count must be the scene count");
+    SurrogateState &S = *sur_;
+    if (S.rep.check_steps == 0 || S.sceneErr.n < 2 * count) {
+      for (uint64_t k = 0; k < count; ++k) out[k] = 0.0;
+      return;
+    }
+    std::vector<double> v(2 * count);
+    NBS_HIP(hipMemcpyAsync(v.data(), S.sceneErr.p, v.size() * 8, hipMemcpyDeviceToHost, stream_));
+    NBS_HIP(hipStreamSynchronize(stream_));
+    for (uint64_t k = 0; k < count; ++k) out[k] = v[2 * k + 1] != 0.0 ? v[2 * k] / std::fabs(v[2 * k + 1]) : 0.0;
   }
   nbs_surrogate_report surReport() {
     if (!sur_) return nbs_surrogate_report{};
@@ -981,8 +1005,9 @@ class Simulation {
     DevBuf<uint16_t> w1b, w2b, w3b;
     DevBuf<float2> target, shadowX, shadowV, densA, densB, presScratch;
     DevBuf<uint32_t> candScratch, pairPart;
-    DevBuf<double> errPartials, errOut;
+    DevBuf<double> errPartials, errOut, sceneErr;
     size_t inferCap = 0, trainCap = 0;
+    float onlineLr = 0.0f;  // > 0: check steps also train on their plain frame
     int pairWaves = 0;  // pair counts written by the last network evaluation
     bool normSet = false;
     int mode = NBS_RELAX_PLAIN;
@@ -1022,6 +1047,10 @@ class Simulation {
     S.errOut.alloc(2);
     S.pairPart.alloc(surrogate_forward_waves((int)c));
     S.inferCap = c;
+  }
+  void ensureSceneErr() {
+    SurrogateState &S = *sur_;
+    if (S.sceneErr.n < 2 * scenes_.size()) S.sceneErr.alloc(2 * scenes_.size());
   }
   void ensureSurTraining() {
     SurrogateState &S = *sur_;
@@ -1073,15 +1102,19 @@ class Simulation {
     const int *cap = tuning_.tile_cap;
     switch (relax) {
       case RELAX_SURROGATE:
-        launch_surrogate_forward(buf_, p, s0, n, surBufs(false, 0), 1, stream_);
+        launch_surrogate_forward(buf_, p, s0, n, surBufs(false, 0), kFwdState, stream_);
         break;
       case RELAX_CHECK: {
-        // plain result into the shadow state, the network's into the state
+        // plain result into the shadow state, the network's into the state;
+        // with online training the plain displacement is also the target
+        const bool online = S.onlineLr > 0.0f;
         Buffers bp = buf_;
         bp.x0 = S.shadowX.p;
         bp.v = S.shadowV.p;
+        if (online) bp.dxOut = S.target.p;
         launch_delta(bp, p, s0, n, cap[2], stream_);
-        launch_surrogate_forward(buf_, p, s0, n, surBufs(false, 0), 1, stream_);
+        const SurrogateBufs m = surBufs(online, 0);
+        launch_surrogate_forward(buf_, p, s0, n, m, kFwdState | (online ? kFwdTrain : 0), stream_);
         // density of both results on this step's grid
         Buffers bd = buf_;
         bd.pres = S.presScratch.p;
@@ -1093,6 +1126,15 @@ class Simulation {
         bd.dens = S.densB.p;
         launch_density(bd, p, s0, n, cap[1], stream_);
         launch_density_error(S.densA.p, S.densB.p, n, S.errPartials.p, S.errOut.p, stream_);
+        launch_density_error_scenes(S.densA.p, S.densB.p, buf_.keys, buf_.sceneBase, buf_.nScenes, n,
+                                    S.sceneErr.p, stream_);
+        if (online) {
+          launch_surrogate_backward(buf_, p, s0, n, m, stream_);
+          launch_adam(S.params.p, S.grad.p, S.adamM.p, S.adamV.p, kNumWeights, S.onlineLr, 0.9f, 0.999f, 1.0e-8f,
+                      (int)++S.adamT, stream_);
+          launch_bf16_weights(S.params.p, S.w1b.p, S.w2b.p, S.w3b.p, stream_);
+          ++S.rep.train_steps;
+        }
         break;
       }
       case RELAX_TRAIN:
@@ -1103,7 +1145,7 @@ class Simulation {
         launch_delta(bt, p, s0, n, cap[2], stream_);
         if (relax == RELAX_CALIBRATE) break;
         const SurrogateBufs m = surBufs(true, lossIndex);
-        launch_surrogate_forward(buf_, p, s0, n, m, 2, stream_);
+        launch_surrogate_forward(buf_, p, s0, n, m, kFwdTrain, stream_);
         launch_surrogate_backward(buf_, p, s0, n, m, stream_);
         if (relax == RELAX_TRAIN) {
           launch_adam(S.params.p, S.grad.p, S.adamM.p, S.adamV.p, kNumWeights, S.lr, 0.9f, 0.999f, 1.0e-8f,
@@ -1575,6 +1617,16 @@ nbs_status nbs_surrogate_train(nbs_sim *sim, float dt, uint32_t steps, float lea
 nbs_status nbs_surrogate_set_mode(nbs_sim *sim, int32_t mode, uint32_t check_interval, float max_density_error) {
   NBS_CHECK_SIM(sim);
   return guarded([&] { sim->impl->surSetMode(mode, check_interval, max_density_error); });
+}
+
+nbs_status nbs_surrogate_set_online_training(nbs_sim *sim, float learning_rate) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->surSetOnlineTraining(learning_rate); });
+}
+
+nbs_status nbs_surrogate_get_scene_errors(nbs_sim *sim, double *errors, uint64_t count) {
+  NBS_CHECK_SIM(sim);
+  return guarded([&] { sim->impl->surSceneErrors(errors, count); });
 }
 
 nbs_status nbs_surrogate_get_report(nbs_sim *sim, nbs_surrogate_report *report) {

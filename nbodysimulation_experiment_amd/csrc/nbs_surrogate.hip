@@ -335,7 +335,7 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
       ohi = mfma(a[s], load_wb(W, kHid, 32, 8 * s), ohi);
     }
   };
-  const bool train = (mode == 2);
+  const bool train = (mode & kFwdTrain) != 0;
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const floatx16 &hlo = t ? h10 : h00;
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
         m.out[2 * (size_t)i] = o;
         m.out[2 * (size_t)i + 1] = oy;
       }
-      if (mode == 1) {
+      if (mode & kFwdState) {
         const uint32_t key = b.keys[i];
         const DevScene sc = scene_of_slot<MULTI>(b, s0, key);
         const float2 xi = b.x1[i];
@@ -701,6 +701,43 @@ __global__ __launch_bounds__(kBlock) void k_dens_err_partial(const float2 *a, co
   }
 }
 
+// Per-scene sums of |a.x - b.x| and b.x into out[2 s], out[2 s + 1] (zeroed
+// beforehand).  A scene's particles occupy consecutive slots, so a wave
+// whose lanes share one scene reduces first and adds once.
+__global__ __launch_bounds__(kBlock) void k_dens_err_scenes(const float2 *a, const float2 *bb, const uint32_t *keys,
+                                                            const uint32_t *sceneBase, int nScenes, int n,
+                                                            double *out) {
+  const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+  const bool valid = i < n;
+  double e = 0.0, r = 0.0;
+  int sc = -1;
+  if (valid) {
+    e = fabs((double)a[i].x - (double)bb[i].x);
+    r = (double)bb[i].x;
+    sc = scene_of_key(sceneBase, nScenes, keys[i]);
+  }
+  const int first = __shfl(sc, 0, 64);
+  const bool uniform = __ballot(valid && sc != first) == 0ull;
+  if (uniform) {
+    for (int off = 32; off > 0; off >>= 1) {
+      e += __shfl_xor(e, off, 64);
+      r += __shfl_xor(r, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && first >= 0) {
+      atomicAdd(&out[2 * first], e);
+      atomicAdd(&out[2 * first + 1], r);
+    }
+  } else if (valid) {
+    atomicAdd(&out[2 * sc], e);
+    atomicAdd(&out[2 * sc + 1], r);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_f64(double *p, int count) {
+  const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (i < count) p[i] = 0.0;
+}
+
 __global__ void k_dens_err_final(const double *partials, int nb, double *out) {
   if (threadIdx.x != 0) return;
   double t0 = 0.0, t1 = 0.0;
@@ -803,6 +840,14 @@ void launch_density_error(const float2 *a, const float2 *bb, int n, double *part
   const int nb = std::max(1, (n + kBlock * kErrItems - 1) / (kBlock * kErrItems));
   if (n > 0) hipLaunchKernelGGL(k_dens_err_partial, dim3(nb), dim3(kBlock), 0, s, a, bb, n, partials);
   hipLaunchKernelGGL(k_dens_err_final, dim3(1), dim3(64), 0, s, (const double *)partials, n > 0 ? nb : 0, out);
+}
+
+void launch_density_error_scenes(const float2 *a, const float2 *bb, const uint32_t *keys, const uint32_t *sceneBase,
+                                 int nScenes, int n, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_f64, dim3((2 * nScenes + kBlock - 1) / kBlock), dim3(kBlock), 0, s, out, 2 * nScenes);
+  if (n > 0)
+    hipLaunchKernelGGL(k_dens_err_scenes, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, bb, keys, sceneBase,
+                       nScenes, n, out);
 }
 
 void launch_mfma_probe(const float *a, const float *b, float *c, hipStream_t s) {

@@ -69,9 +69,11 @@ struct SurrogateBufs {
   uint32_t *pairPart;        // per forward wave: pairs processed (may be null)
 };
 
-// Evaluates the network on the step's grid.  mode 0: outputs only (out);
-// mode 1: relaxation step (x0, v written as the plain relaxation does);
-// mode 2: training forward (out, act, cnt).
+// Evaluates the network on the step's grid.  mode flags: kFwdState writes the
+// relaxation step's result (x0, v, as the plain relaxation does); kFwdTrain
+// writes the training activations (out, act, cnt); out is written when set.
+constexpr int kFwdState = 1;
+constexpr int kFwdTrain = 2;
 void launch_surrogate_forward(const Buffers &b, const DevParams &p, const DevScene &s0, int n,
                               const SurrogateBufs &m, int mode, hipStream_t stream);
 // Gradient of L for the frame of the last training forward into m.grad
@@ -90,6 +92,10 @@ void launch_adam(float *params, const float *grad, float *m, float *v, int count
 // diagnostics_blocks(n) x 2.
 void launch_density_error(const float2 *a, const float2 *b, int n, double *partials, double *out,
                           hipStream_t stream);
+// Per-scene FP64 sums: out[2 s] = sum |a.x - b.x|, out[2 s + 1] = sum b.x over
+// the slots of scene s (found from the slot's cell key).
+void launch_density_error_scenes(const float2 *a, const float2 *b, const uint32_t *keys, const uint32_t *sceneBase,
+                                 int nScenes, int n, double *out, hipStream_t stream);
 // C (32 x 32) = A (32 x 8) B (8 x 32), row-major FP32 inputs rounded to BF16.
 void launch_mfma_probe(const float *a, const float *b, float *c, hipStream_t stream);
 

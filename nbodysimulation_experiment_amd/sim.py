@@ -322,6 +322,16 @@ class ParticleSimulation:
         mode = _lib.RELAX_SURROGATE if surrogate else _lib.RELAX_PLAIN
         call("nbs_surrogate_set_mode", self._h, mode, int(check_interval), float(max_density_error))
 
+    def surrogate_set_online_training(self, learning_rate: float):
+        """Check steps in surrogate mode also train on their plain frame (0: off)."""
+        call("nbs_surrogate_set_online_training", self._h, float(learning_rate))
+
+    def surrogate_scene_errors(self) -> np.ndarray:
+        """Per-scene density errors of the last check step."""
+        out = np.zeros(self.GetSceneCount(), np.float64)
+        call("nbs_surrogate_get_scene_errors", self._h, _ptr(out), out.size)
+        return out
+
     def surrogate_report(self) -> dict:
         r = NbsSurrogateReport()
         call("nbs_surrogate_get_report", self._h, ctypes.byref(r))

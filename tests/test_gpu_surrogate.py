@@ -175,3 +175,49 @@ def test_surrogate_mode_argument_errors():
             sim.surrogate_set_params(np.zeros(10, np.float32))
         with pytest.raises(NbsError):
             sim.surrogate_train(DT, 1, -1.0)
+
+
+def test_batched_surrogate_with_in_run_training_and_scene_errors():
+    """Uneven batched scenes plus one larger scene (the batched configuration
+    at reduced size): surrogate inference with fine-tuning on each check
+    step's plain frame, and per-scene density errors."""
+    from nbodysimulation_experiment_amd import scenes
+    specs = scenes.batched_scenes(seed=5, count=24, n_min=2048, n_max=16384)
+    big = volume_positions((-30.0, -30.0), 256, 256, 0.5, 1.0, seed=77)
+    specs.append(SceneSpec(200.0, 200.0, 1.0, big, planes=wall_planes(200.0, 200.0)))
+    with ParticleSimulation.from_scenes(specs, SPHParams(), GRAVITY) as sim:
+        sim.Update(DT, 40)
+        sim.surrogate_init(2)
+        sim.surrogate_calibrate(DT)
+        sim.surrogate_train(DT, 30, 1e-3)
+        p0 = sim.surrogate_params()
+        sim.surrogate_set_mode(True, 4, 1e30)  # check every 4th step, never fall back
+        sim.surrogate_set_online_training(1e-3)
+        before = sim.surrogate_report()["train_steps"]
+        sim.Update(DT, 32)
+        r = sim.surrogate_report()
+        errs = sim.surrogate_scene_errors()
+        st = sim.get_state()
+        p1 = sim.surrogate_params()
+    print(f"checks {r['check_steps']} train steps {r['train_steps'] - before} last error {r['last_density_error']:.3e} "
+          f"scene errors {errs.min():.3e}..{errs.max():.3e}", flush=True)
+    assert r["check_steps"] == 8 and r["fallbacks"] == 0
+    assert r["train_steps"] - before == 8          # one Adam step per check
+    assert not np.array_equal(p0[:so().NUM_WEIGHTS], p1[:so().NUM_WEIGHTS])
+    assert np.array_equal(p0[so().OFF_NORM:], p1[so().OFF_NORM:])
+    assert errs.shape == (len(specs),) and np.isfinite(errs).all() and (errs > 0).all()
+    # the global error is the plain-density-weighted mean of the scene errors
+    assert errs.min() * (1 - 1e-9) <= r["last_density_error"] <= errs.max() * (1 + 1e-9)
+    assert np.isfinite(st["x"]).all()
+
+
+def test_single_scene_error_equals_global_error():
+    with developed_4k(40) as sim:
+        sim.surrogate_init(4)
+        sim.surrogate_calibrate(DT)
+        sim.surrogate_set_mode(True, 2, 1e30)
+        sim.Update(DT, 4)
+        r = sim.surrogate_report()
+        errs = sim.surrogate_scene_errors()
+    assert errs.shape == (1,)
+    assert abs(errs[0] - r["last_density_error"]) <= 1e-9 * r["last_density_error"]
