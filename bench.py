@@ -108,12 +108,72 @@ def train_surrogate(args):
                                                  "10% of the steps)"}
 
 
+def scenario_benchmark(args):
+    """The reference application's built-in benchmark (64 frames x 16
+    iterations per scenario, per-phase min / max / avg) on the GPU, beside the
+    reference-order multi-threaded CPU step on the same frames (fewer
+    iterations, since its cost is the same in every iteration)."""
+    import numpy as np
+
+    from nbodysimulation_experiment_amd import scenarios as sc
+    from nbodysimulation_experiment_amd.scenes import volume_positions
+    from oracle.oracle import GS_MT, OracleSim
+    threads = args.cpu_threads or min(16, int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1)
+    dt = float(sc.CONSTANTS["delta_time"])
+    rows = []
+    for idx, s in enumerate(sc.SCENARIOS):
+        with sc.make_simulation(s) as sim:
+            g = sc.run_benchmark(sim, idx, frames=sc.BENCHMARK_FRAMES, iterations=sc.BENCHMARK_ITERATIONS)
+        cpu_ms = []
+        for _ in range(args.scenario_cpu_iterations):
+            o = OracleSim(sc.scenario_params(s), 32, GS_MT, threads)
+            c = sc.CONSTANTS
+            scene = o.add_scene(float(c["boundary_width"]), float(c["boundary_height"]), float(c["grid_cell_size"]))
+            o.set_spacing(s.params["particle_spacing"])
+            o.set_gravity(*map(float, s.gravity))
+            for b in s.bodies:
+                r = sc.resolve_body(b)
+                if r["kind"] == "plane":
+                    o.add_plane(scene, r["normal"][0], r["normal"][1], r["distance"])
+                elif r["kind"] == "circle":
+                    o.add_circle(scene, r["center"][0], r["center"][1], r["radius"])
+                else:
+                    o.add_polygon(scene, np.array(r["verts"], np.float32))
+            for v in s.volumes:
+                nx, ny = sc.volume_counts(v, s.params["particle_spacing"])
+                pos = volume_positions(v.position, nx, ny, float(s.params["particle_spacing"]),
+                                       float(s.params["kernel_height"]), 0)
+                o.add_particles(scene, pos, np.tile(np.array(v.force, np.float32), (pos.shape[0], 1)))
+            for e in s.emitters:
+                o.add_emitter(scene, e.position, e.direction, e.radius, e.speed, e.rate, e.duration, seed=0)
+            for _ in range(sc.BENCHMARK_FRAMES):
+                t0 = time.perf_counter()
+                o.step(dt, 1)
+                cpu_ms.append((time.perf_counter() - t0) * 1e3)
+            o.close()
+        row = {"scenario": s.name, "particles_max": int(g["particles"][1]),
+               "gpu_ms": [round(x, 4) for x in g["simulation_ms"]],
+               "gpu_phases_avg_ms": {k: round(v[2], 4) for k, v in g["phases_ms"].items() if v[2] > 0},
+               "cpu_ms": [round(min(cpu_ms), 4), round(max(cpu_ms), 4), round(sum(cpu_ms) / len(cpu_ms), 4)]}
+        row["cpu_over_gpu_avg"] = round(row["cpu_ms"][2] / row["gpu_ms"][2], 2)
+        rows.append(row)
+        log(f"{s.name}: GPU avg {row['gpu_ms'][2]:.3f} ms/frame, CPU ({threads} threads) avg {row['cpu_ms'][2]:.3f}")
+    out = {"metric": "built-in benchmark: ms per frame (min, max, avg) per scenario", "unit": "ms",
+           "higher_is_better": False, "frames": sc.BENCHMARK_FRAMES, "iterations": sc.BENCHMARK_ITERATIONS,
+           "cpu": {"kind": "port", "cores": threads, "iterations": args.scenario_cpu_iterations,
+                   "order": "reference (Gauss-Seidel, pooled phases)"},
+           "dt": dt, "scenarios": rows}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="dam-break-1m", choices=["dam-break-4k", "dam-break-1m", "batched-256"])
+    ap.add_argument("--workload", default="dam-break-1m",
+                    choices=["dam-break-4k", "dam-break-1m", "batched-256", "scenarios"])
+    ap.add_argument("--scenario-cpu-iterations", type=int, default=2)
     ap.add_argument("--graph", action="store_true", help="replay steps as HIP graphs (no per-kernel timing)")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -135,6 +195,10 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build computes on one GPU; runs on more than one GPU are not supported", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "scenarios":
+        scenario_benchmark(args)
+        return
 
     from nbodysimulation_experiment_amd.sim import ParticleSimulation
     from nbodysimulation_experiment_amd.scenes import SPHParams
