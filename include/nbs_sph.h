@@ -211,7 +211,10 @@ nbs_status nbs_set_options(nbs_sim *sim, uint32_t flags);
 typedef struct nbs_tuning {
   int32_t autotune;
   uint32_t retune_interval;
-  int32_t tile_cap[3]; /* viscosity, density, relaxation: 0, 1024, 2048 or 4096 */
+  int32_t tile_cap[3]; /* viscosity, density, relaxation: LDS tile capacity 0, 1024, 2048 or 4096, or
+                          -1 for the wide kernels (one wave per particle); all give identical results */
+  int32_t wide_max_particles; /* steps with at most this many particles use the wide kernels for all three
+                                 phases whatever tile_cap says; 0 (default) leaves the choice to tile_cap */
 } nbs_tuning;
 nbs_status nbs_set_tuning(nbs_sim *sim, const nbs_tuning *tuning);
 nbs_status nbs_get_tuning(nbs_sim *sim, nbs_tuning *tuning);

@@ -65,7 +65,8 @@ class NbsDiagnostics(ctypes.Structure):
 
 
 class NbsTuning(ctypes.Structure):
-    _fields_ = [("autotune", c_int32), ("retune_interval", c_uint32), ("tile_cap", c_int32 * 3)]
+    _fields_ = [("autotune", c_int32), ("retune_interval", c_uint32), ("tile_cap", c_int32 * 3),
+                ("wide_max_particles", c_int32)]
 
 
 class NbsSurrogateReport(ctypes.Structure):

@@ -61,6 +61,48 @@ __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const D
   return cand;
 }
 
+// Wide pair walk for small particle counts: one wave per particle.  Lane t
+// tests candidate base + t of a 64-candidate chunk and, when the pair is in
+// range, term() computes the pair's K terms and says whether it contributes.
+// The contributions are then applied by add() in candidate order, one lane's
+// terms at a time (broadcast with readlane), so the accumulation performs the
+// same FP32 operations in the same order as for_each_pair; only the per-pair
+// arithmetic runs in parallel across the wave.  Returns the candidate count.
+template <int K, typename Idx, typename Term, typename Add>
+__device__ __forceinline__ uint32_t for_each_pair_wide(const uint32_t *start, const DevScene &sc, int cx, int cy,
+                                                       const float2 *X, Idx idx, float2 xi, float h2, Term &&term,
+                                                       Add &&add) {
+  const int lane = threadIdx.x & 63;
+  uint32_t cand = 0;
+  for (int d = 0; d < 3; ++d) {
+    const RowRange rr = row_range(start, sc, cx, cy, d - 1);
+    cand += rr.j1 - rr.j0;
+    for (uint32_t base = rr.j0; base < rr.j1; base += 64u) {
+      const uint32_t j = base + (uint32_t)lane;
+      float t[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) t[k] = 0.0f;
+      bool ok = false;
+      if (j < rr.j1) {
+        const float2 xj = X[idx(d, base) + (uint32_t)lane];
+        const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;
+        const float r2 = Rx * Rx + Ry * Ry;
+        if (r2 < h2) ok = term(j, Rx, Ry, r2, t);
+      }
+      uint64_t mask = __ballot(ok);
+      while (mask) {
+        const int src = __builtin_ctzll(mask);
+        mask &= mask - 1u;
+        float v[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t[k]), src));
+        add(v);
+      }
+    }
+  }
+  return cand;
+}
+
 // Arrival index of this lane's particle in cell `key`: lanes of the wave with
 // equal keys are grouped, the lowest lane of each group adds the group size to
 // the cell counter with one atomic, and each lane takes base + its rank in the
@@ -452,6 +494,171 @@ __global__ __launch_bounds__(kBlock) void k_delta(Buffers b, DevParams p, DevSce
 }
 
 // ---------------------------------------------------------------------------
+// Wide variants for small particle counts: one wave per particle, the pair
+// terms computed across the wave and accumulated in candidate order
+// (for_each_pair_wide), so results equal the one-lane-per-particle kernels'
+// bit for bit.  Used when a step has at most nbs_tuning.wide_max_particles
+// particles: there, one lane per particle leaves most of the chip idle and
+// each particle's serial candidate walk sets the kernel time.
+// ---------------------------------------------------------------------------
+constexpr int kWidePerBlock = kBlock / 64;
+
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_visc_predict_wide(Buffers b, DevParams p, DevScene s0, int n, int nBinned,
+                                                              int lists) {
+  const int i = (int)(blockIdx.x * kWidePerBlock + (threadIdx.x >> 6));
+  if (i >= n) return;  // whole wave
+  const float2 xi = b.x0[i];
+  const float2 vi = b.v[i];
+  const bool binned = i < nBinned;
+  float aX, aY;
+  DevScene sc;
+  if (binned) {
+    aX = p.gex + 0.0f;                                   // demo4.cpp:306 (own acceleration is zero)
+    aY = p.gey + 0.0f;
+    sc = scene_of_slot<MULTI>(b, s0, b.keys[i]);
+  } else {
+    const float2 a = b.acc[i];
+    aX = p.gex + a.x;
+    aY = p.gey + a.y;
+    sc = MULTI ? b.scenes[b.scn[i]] : s0;
+  }
+  float vx = (aX * p.dt) + vi.x;                         // demo4.cpp:307
+  float vy = (aY * p.dt) + vi.y;
+  if (lists && binned) {
+    const uint32_t local = b.keys[i] - sc.cellBase;
+    const int cy = (int)(local / (uint32_t)sc.nx);
+    const int cx = (int)local - cy * sc.nx;
+    float accx = 0.0f, accy = 0.0f;
+    const float2 *V = b.v;
+    for_each_pair_wide<2>(
+        b.startCur, sc, cx, cy, b.x0, SlotIndex(), xi, p.h2,
+        [&](uint32_t j, float Rx, float Ry, float r2, float *t) {
+          const float2 vj = V[j];
+          const float vjx = p.gdtx + vj.x, vjy = p.gdty + vj.y;
+          const float r = sqrt_rn(r2);                          // sph.h:503
+          const float qq = r * p.invH;                          // sph.h:504
+          const float inv = (r == 0.0f) ? 1.0f : rcp_rn(r);     // sph.h:505
+          const float nX = Rx * inv, nY = Ry * inv;
+          const float u = (vx - vjx) * nX + (vy - vjy) * nY;    // sph.h:506
+          if (!(u > 0.0f)) return false;
+          const float f = (1.0f - qq) * (p.sigma * u + p.beta * (u * u));  // sph.h:508
+          t[0] = ((f * nX) * 0.5f) * p.dt;                      // demo4.cpp:233
+          t[1] = ((f * nY) * 0.5f) * p.dt;
+          return true;
+        },
+        [&](const float *v) {
+          accx = accx - v[0];                                   // own visit
+          accy = accy - v[1];
+          accx = accx - v[0];                                   // neighbour's visit
+          accy = accy - v[1];
+        });
+    vx = accx + vx;
+    vy = accy + vy;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    const float xpx = (vx * p.dt) + xi.x;                // demo4.cpp:334-335
+    const float xpy = (vy * p.dt) + xi.y;
+    b.xp[i] = make_float2(xpx, xpy);
+    const int cx = cell_coord(xpx, sc.halfW, sc.cellSize, sc.nx);
+    const int cy = cell_coord(xpy, sc.halfH, sc.cellSize, sc.ny);
+    const uint32_t key = sc.cellBase + (uint32_t)(cy * sc.nx + cx);
+    b.keyp[i] = key;
+    b.fill[i] = atomicAdd(&b.count[key], 1u);  // arrival slot; the rank pass orders the cell
+  }
+}
+
+// (min, max) candidate-count pairs of the one-lane-per-particle workgroups,
+// reset before the wide density kernel folds its counts in.
+__global__ __launch_bounds__(kBlock) void k_reset_minmax(uint32_t *mm, int pairs) {
+  const int k = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (k < pairs) {
+    mm[2 * k] = 0xFFFFFFFFu;
+    mm[2 * k + 1] = 0u;
+  }
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_density_wide(Buffers b, DevParams p, DevScene s0, int n) {
+  const int i = (int)(blockIdx.x * kWidePerBlock + (threadIdx.x >> 6));
+  if (i >= n) return;  // whole wave
+  const uint32_t key = b.keys[i];
+  const DevScene sc = scene_of_slot<MULTI>(b, s0, key);
+  const uint32_t local = key - sc.cellBase;
+  const int cy = (int)(local / (uint32_t)sc.nx);
+  const int cx = (int)local - cy * sc.nx;
+  const float2 xi = b.x1[i];
+  float d0 = 0.0f, d1 = 0.0f;
+  const uint32_t cand = for_each_pair_wide<2>(
+      b.startNext, sc, cx, cy, b.x1, SlotIndex(), xi, p.h2,
+      [&](uint32_t, float, float, float r2, float *t) {
+        const float r = sqrt_rn(r2);                            // sph.h:471
+        const float term = 1.0f - r * p.invH;                   // sph.h:472
+        t[0] = term * term;
+        t[1] = (term * term) * term;
+        return true;
+      },
+      [&](const float *v) {
+        d0 = d0 + v[0];                                         // sph.h:473
+        d1 = d1 + v[1];                                         // sph.h:474
+      });
+  if ((threadIdx.x & 63) == 0) {
+    b.dens[i] = make_float2(d0, d1);
+    b.pres[i] = make_float2(p.k * (d0 - p.rho0), p.kn * d1);   // sph.h:478-481
+    const int w = i / kBlock;                                   // statistics slot of the narrow layout
+    atomicMin(&b.candStats[2 * w], cand);
+    atomicMax(&b.candStats[2 * w + 1], cand);
+  }
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_delta_wide(Buffers b, DevParams p, DevScene s0, int n) {
+  const int i = (int)(blockIdx.x * kWidePerBlock + (threadIdx.x >> 6));
+  if (i >= n) return;  // whole wave
+  const uint32_t key = b.keys[i];
+  const DevScene sc = scene_of_slot<MULTI>(b, s0, key);
+  const uint32_t local = key - sc.cellBase;
+  const int cy = (int)(local / (uint32_t)sc.nx);
+  const int cx = (int)local - cy * sc.nx;
+  const float2 xi = b.x1[i];
+  const float2 pi = b.pres[i];
+  const float2 *P = b.pres;
+  float accx = 0.0f, accy = 0.0f;
+  for_each_pair_wide<4>(
+      b.startNext, sc, cx, cy, b.x1, SlotIndex(), xi, p.h2,
+      [&](uint32_t j, float Rx, float Ry, float r2, float *t) {
+        const float2 pj = P[j];
+        const float r = sqrt_rn(r2);                            // sph.h:489
+        const float inv = (r == 0.0f) ? 1.0f : rcp_rn(r);       // sph.h:490
+        const float nX = Rx * inv, nY = Ry * inv;
+        const float term = 1.0f - r * p.invH;                   // sph.h:491
+        const float tt = term * term;
+        const float dij = p.dt2 * (pi.x * term + pi.y * tt);    // sph.h:492 with P_i
+        const float dji = p.dt2 * (pj.x * term + pj.y * tt);    // sph.h:492 with P_j
+        t[0] = (dij * nX) * 0.5f;
+        t[1] = (dij * nY) * 0.5f;
+        t[2] = (dji * nX) * 0.5f;
+        t[3] = (dji * nY) * 0.5f;
+        return true;
+      },
+      [&](const float *v) {
+        accx = accx - v[0];                                     // demo4.cpp:251
+        accy = accy - v[1];
+        accx = accx - v[2];                                     // demo4.cpp:250 (j's visit)
+        accy = accy - v[3];
+      });
+  if ((threadIdx.x & 63) == 0) {
+    if (b.dxOut) b.dxOut[i] = make_float2(accx, accy);
+    float x = accx + xi.x;                                      // demo4.cpp:253
+    float y = accy + xi.y;
+    collide(b, p, sc, x, y);                                    // demo4.cpp:411-444
+    const float2 pv = b.prev[i];
+    b.x0[i] = make_float2(x, y);
+    b.v[i] = make_float2((x - pv.x) * p.invDt, (y - pv.y) * p.invDt);  // demo4.cpp:449
+  }
+}
+
+// ---------------------------------------------------------------------------
 // State readback in particle-index order
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_gather_state(Buffers b, DevScene s0, int n, int nBinned, float2 *x,
@@ -613,9 +820,19 @@ static void visc_cap(const Buffers &b, const DevParams &p, const DevScene &s0, i
                        lists);
 }
 
+static unsigned wide_grid(int n) { return (unsigned)((n + kWidePerBlock - 1) / kWidePerBlock); }
+
 void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int nBinned, int lists,
                          int cap, hipStream_t s) {
   if (n <= 0) return;
+  if (cap == kWideCap) {
+    if (b.nScenes > 1)
+      hipLaunchKernelGGL(k_visc_predict_wide<true>, dim3(wide_grid(n)), dim3(kBlock), 0, s, b, p, s0, n, nBinned, lists);
+    else
+      hipLaunchKernelGGL(k_visc_predict_wide<false>, dim3(wide_grid(n)), dim3(kBlock), 0, s, b, p, s0, n, nBinned,
+                         lists);
+    return;
+  }
   switch (cap) {
     case 0: visc_cap<0>(b, p, s0, n, nBinned, lists, s); break;
     case 1024: visc_cap<1024>(b, p, s0, n, nBinned, lists, s); break;
@@ -646,6 +863,15 @@ static void density_cap(const Buffers &b, const DevParams &p, const DevScene &s0
 
 void launch_density(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int cap, hipStream_t s) {
   if (n <= 0) return;
+  if (cap == kWideCap) {
+    const int pairs = (int)grid_for(n);
+    hipLaunchKernelGGL(k_reset_minmax, dim3((pairs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b.candStats, pairs);
+    if (b.nScenes > 1)
+      hipLaunchKernelGGL(k_density_wide<true>, dim3(wide_grid(n)), dim3(kBlock), 0, s, b, p, s0, n);
+    else
+      hipLaunchKernelGGL(k_density_wide<false>, dim3(wide_grid(n)), dim3(kBlock), 0, s, b, p, s0, n);
+    return;
+  }
   switch (cap) {
     case 0: density_cap<0>(b, p, s0, n, s); break;
     case 1024: density_cap<1024>(b, p, s0, n, s); break;
@@ -664,6 +890,13 @@ static void delta_cap(const Buffers &b, const DevParams &p, const DevScene &s0, 
 
 void launch_delta(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int cap, hipStream_t s) {
   if (n <= 0) return;
+  if (cap == kWideCap) {
+    if (b.nScenes > 1)
+      hipLaunchKernelGGL(k_delta_wide<true>, dim3(wide_grid(n)), dim3(kBlock), 0, s, b, p, s0, n);
+    else
+      hipLaunchKernelGGL(k_delta_wide<false>, dim3(wide_grid(n)), dim3(kBlock), 0, s, b, p, s0, n);
+    return;
+  }
   switch (cap) {
     case 0: delta_cap<0>(b, p, s0, n, s); break;
     case 1024: delta_cap<1024>(b, p, s0, n, s); break;

@@ -85,8 +85,12 @@ void launch_scan(const Buffers &b, uint32_t cells, hipStream_t stream);
 void launch_scatter_reorder(const Buffers &b, int n, hipStream_t stream);
 void launch_density(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int cap, hipStream_t stream);
 void launch_delta(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int cap, hipStream_t stream);
-constexpr int kNumTileCaps = 4;
-constexpr int kTileCaps[kNumTileCaps] = {0, 1024, 2048, 4096};
+// cap value selecting the wide kernels (one wave per particle) of the three
+// pair launchers; results are bit-identical to the other capacities.
+constexpr int kWideCap = -1;
+// The variants the autotuner orders and times (the wide kernels last).
+constexpr int kNumTileCaps = 5;
+constexpr int kTileCaps[kNumTileCaps] = {0, 1024, 2048, 4096, kWideCap};
 void launch_gather_state(const Buffers &b, const DevScene &s0, int n, int nBinned, float2 *x, float2 *prev, float2 *v,
                          float2 *dens, float2 *pres, uint32_t *key, int32_t *scene, hipStream_t stream);
 void launch_colors(const Buffers &b, float rho0, int n, float4 *rgba, hipStream_t stream);

@@ -386,11 +386,10 @@ class Simulation {
         ++s;
         continue;
       }
-      if (wantTune() && nsteps - s >= tuneSteps()) {
-        const uint32_t done = tune(dt, slots);
-        s += done;
-        slots += done;
-        stepsSinceTune_ = 0;
+      if ((tuneActive_ || wantTune()) && !surrogateMode()) {
+        tuneStep(dt, slots);  // one timed step of the next candidate
+        ++slots;
+        ++s;
         continue;
       }
       const bool graphable = (options_ & NBS_OPT_HIP_GRAPH) && !timing_ && emitters_.empty() && lists_ &&
@@ -431,6 +430,7 @@ class Simulation {
     }
     NBS_HIP(hipGetLastError());
     NBS_HIP(hipStreamSynchronize(stream_));
+    if (!tunePending_.empty()) absorbTuneSlots();  // this call's tuning steps (event slots are per call)
     if (timing_ && slots > 0) collectTimes(slots);
     statsPending_ = true;
     if (options_ & NBS_OPT_CHECK_FINITE) {
@@ -441,10 +441,14 @@ class Simulation {
 
   void setTuning(const nbs_tuning &t) {
     for (int k = 0; k < 3; ++k)
-      if (t.tile_cap[k] != 0 && t.tile_cap[k] != 1024 && t.tile_cap[k] != 2048 && t.tile_cap[k] != 4096)
-        throw Error(NBS_ERR_INVALID_ARGUMENT, "tile capacities are 0, 1024, 2048 or 4096");
+      if (t.tile_cap[k] != 0 && t.tile_cap[k] != 1024 && t.tile_cap[k] != 2048 && t.tile_cap[k] != 4096 &&
+          t.tile_cap[k] != kWideCap)
+        throw Error(NBS_ERR_INVALID_ARGUMENT, "tile capacities are 0, 1024, 2048, 4096 or -1 (wide)");
+    if (t.wide_max_particles < 0) throw Error(NBS_ERR_INVALID_ARGUMENT, "wide_max_particles must be >= 0");
     tuning_ = t;
     tuned_ = false;
+    tuneActive_ = false;
+    tunePending_.clear();
     stepsSinceTune_ = 0;
     destroyGraph();
   }
@@ -800,13 +804,21 @@ class Simulation {
     destroyGraph();
   }
 
+  // Pair-kernel variant per phase (viscosity, density, relaxation): an LDS
+  // tile capacity, or kWideCap for small steps.
+  bool wideStep() const { return tuning_.wide_max_particles > 0 && n_ <= (uint64_t)tuning_.wide_max_particles; }
+  void pairCaps(int cap[3]) const {
+    for (int k = 0; k < 3; ++k) cap[k] = wideStep() ? kWideCap : tuning_.tile_cap[k];
+  }
+
   // One step on stream_ (ev >= 0: record timing events for step slot ev);
   // relax selects how the relaxation phase runs (Relax).
   void enqueueStep(float dt, int ev, int relax = RELAX_PLAIN, int lossIndex = 0) {
     const DevParams p = devParams(dt);
     const DevScene s0 = hostScenes_[0];
     const int n = (int)n_;
-    const int *cap = tuning_.tile_cap;
+    int cap[3];
+    pairCaps(cap);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 0], stream_));
     launch_visc_predict(buf_, p, s0, n, (int)nBinned_, lists_ ? 1 : 0, cap[0], stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 1], stream_));
@@ -830,61 +842,78 @@ class Simulation {
   // Each of kTuneReps rounds runs one step per candidate; each kernel keeps
   // its fastest candidate (minimum over rounds).
   static constexpr uint32_t kTuneReps = 2;
-  static constexpr uint64_t kTuneMinParticles = 32768;
-  uint32_t tuneVariants() const { return tuned_ ? 3u : (uint32_t)kNumTileCaps; }
-  uint32_t tuneSteps() const { return kTuneReps * tuneVariants(); }
+  static constexpr uint64_t kTuneMinParticles = 1024;
+  // The wide kernels (one wave per particle) are a candidate only for steps
+  // small enough that one lane per particle leaves the chip idle.
+  static constexpr uint64_t kWideTuneMax = 131072;
+  int numCaps() const { return n_ <= kWideTuneMax ? kNumTileCaps : kNumTileCaps - 1; }
+  uint32_t tuneVariants() const { return tuned_ ? 3u : (uint32_t)numCaps(); }
   bool wantTune() const {
-    if (!tuning_.autotune || n_ < kTuneMinParticles || !lists_ || nBinned_ != n_ || surrogateMode()) return false;
+    if (!tuning_.autotune || n_ < kTuneMinParticles || !lists_ || nBinned_ != n_ || surrogateMode() || wideStep())
+      return false;
     return !tuned_ || (tuning_.retune_interval > 0 && stepsSinceTune_ >= tuning_.retune_interval);
   }
-  static int capIndex(int cap) {
-    for (int v = 0; v < kNumTileCaps; ++v)
+  int capIndex(int cap) const {
+    for (int v = 0; v < numCaps(); ++v)
       if (kTileCaps[v] == cap) return v;
-    return kNumTileCaps - 1;
+    return numCaps() - 1;
   }
-  uint32_t tune(float dt, uint32_t slot0) {
-    const uint32_t V = tuneVariants(), steps = kTuneReps * V;
-    int cand[3][kNumTileCaps];
+  // A tuning run is kTuneReps rounds of one step per candidate, one step per
+  // iteration of the update loop (so emitters still run before every step), and
+  // it may span several update calls: each call reads the times of its own
+  // tuning steps, whose event slots it reuses afterwards.
+  void beginTune() {
+    tuneV_ = tuneVariants();
     for (int k = 0; k < 3; ++k) {
       if (!tuned_) {
-        for (uint32_t v = 0; v < V; ++v) cand[k][v] = kTileCaps[v];
+        for (uint32_t v = 0; v < tuneV_; ++v) tuneCand_[k][v] = kTileCaps[v];
       } else {
         const int c = capIndex(tuning_.tile_cap[k]);
-        cand[k][0] = kTileCaps[std::max(c - 1, 0)];
-        cand[k][1] = kTileCaps[c];
-        cand[k][2] = kTileCaps[std::min(c + 1, kNumTileCaps - 1)];
+        tuneCand_[k][0] = kTileCaps[std::max(c - 1, 0)];
+        tuneCand_[k][1] = kTileCaps[c];
+        tuneCand_[k][2] = kTileCaps[std::min(c + 1, numCaps() - 1)];
       }
+      for (int v = 0; v < kNumTileCaps; ++v) tuneBest_[k][v] = 3.0e38f;
     }
-    ensureEvents(slot0 + steps);
-    for (uint32_t r = 0; r < kTuneReps; ++r) {
-      for (uint32_t v = 0; v < V; ++v) {
-        for (int k = 0; k < 3; ++k) tuning_.tile_cap[k] = cand[k][v];
-        enqueueStep(dt, (int)(slot0 + r * V + v));
-        swapAfterStep();
-      }
-    }
-    NBS_HIP(hipEventSynchronize(events_[5 * (slot0 + steps - 1) + 4]));
-    const int evPair[3][2] = {{0, 1}, {2, 3}, {3, 4}};  // viscosity, density, relaxation
-    for (int k = 0; k < 3; ++k) {
+    tuneDone_ = 0;
+    tunePending_.clear();
+    tuneActive_ = true;
+  }
+  void tuneStep(float dt, uint32_t slot) {
+    if (!tuneActive_) beginTune();
+    const uint32_t v = tuneDone_ % tuneV_;
+    for (int k = 0; k < 3; ++k) tuning_.tile_cap[k] = tuneCand_[k][v];
+    ensureEvents(slot + 1);
+    enqueueStep(dt, (int)slot);
+    swapAfterStep();
+    tunePending_.push_back({slot, v});
+    if (++tuneDone_ < kTuneReps * tuneV_) return;
+    NBS_HIP(hipEventSynchronize(events_[5 * slot + 4]));
+    absorbTuneSlots();
+    for (int k = 0; k < 3; ++k) {  // each kernel keeps its fastest candidate
       float best = 3.0e38f;
-      int choice = cand[k][0];
-      for (uint32_t v = 0; v < V; ++v) {
-        float t = 3.0e38f;
-        for (uint32_t r = 0; r < kTuneReps; ++r) {
-          const uint32_t sl = slot0 + r * V + v;
-          float ms = 0.0f;
-          if (hipEventElapsedTime(&ms, events_[5 * sl + evPair[k][0]], events_[5 * sl + evPair[k][1]]) == hipSuccess)
-            t = std::min(t, ms);
-        }
-        if (t < best) best = t, choice = cand[k][v];
-      }
+      int choice = tuneCand_[k][0];
+      for (uint32_t c = 0; c < tuneV_; ++c)
+        if (tuneBest_[k][c] < best) best = tuneBest_[k][c], choice = tuneCand_[k][c];
       tuning_.tile_cap[k] = choice;
     }
+    tuneActive_ = false;
     tuned_ = true;
     stepsSinceTune_ = 0;
     ++tuneCount_;
     destroyGraph();
-    return steps;
+  }
+  void absorbTuneSlots() {
+    const int evPair[3][2] = {{0, 1}, {2, 3}, {3, 4}};  // viscosity, density, relaxation
+    for (const auto &sv : tunePending_) {
+      for (int k = 0; k < 3; ++k) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, events_[5 * sv.first + evPair[k][0]], events_[5 * sv.first + evPair[k][1]]) ==
+            hipSuccess)
+          tuneBest_[k][sv.second] = std::min(tuneBest_[k][sv.second], ms);
+      }
+    }
+    tunePending_.clear();
   }
 
   void swapAfterStep() {
@@ -1099,7 +1128,8 @@ class Simulation {
   void enqueueLearnedRelaxation(const DevParams &p, const DevScene &s0, int n, int relax, int lossIndex) {
     SurrogateState &S = *sur_;
     if (relax != RELAX_CALIBRATE) S.pairWaves = surrogate_forward_waves(n);
-    const int *cap = tuning_.tile_cap;
+    int cap[3];
+    pairCaps(cap);
     switch (relax) {
       case RELAX_SURROGATE:
         launch_surrogate_forward(buf_, p, s0, n, surBufs(false, 0), kFwdState, stream_);
@@ -1266,8 +1296,13 @@ class Simulation {
   double kernelMs_[NBS_KERNEL_COUNT] = {};
   uint64_t kernelLaunches_[NBS_KERNEL_COUNT] = {};
 
-  nbs_tuning tuning_{1, 64, {2048, 2048, 2048}};
+  nbs_tuning tuning_{1, 64, {2048, 2048, 2048}, 0};
   bool tuned_ = false;
+  bool tuneActive_ = false;
+  uint32_t tuneV_ = 0, tuneDone_ = 0;
+  int tuneCand_[3][kNumTileCaps] = {};
+  float tuneBest_[3][kNumTileCaps] = {};
+  std::vector<std::pair<uint32_t, uint32_t>> tunePending_;  // (event slot, candidate) of this call
   uint64_t stepsSinceTune_ = 0;
   uint64_t tuneCount_ = 0;
 

@@ -179,11 +179,11 @@ def test_dam_break_1m_three_steps_bit_exact():
 
 
 @pytest.mark.parametrize("caps", [(0, 0, 0), (1024, 1024, 1024), (2048, 2048, 2048), (4096, 4096, 4096),
-                                  (0, 4096, 1024)])
+                                  (0, 4096, 1024), (-1, 2048, -1), (-1, -1, -1)])
 def test_tile_capacities_bit_exact(caps):
     spec = dam_break_4k()
     sim = gpu_for([spec])
-    sim.set_tuning(autotune=False, tile_cap=caps)
+    sim.set_tuning(autotune=False, tile_cap=caps, wide_max_particles=0)
     ref = oracle_for([spec])
     sim.Update(DT, 150)
     ref.step(DT, 150)
@@ -200,4 +200,28 @@ def test_autotuned_batched_run_bit_exact():
     ref.step(DT, 60)
     assert_bit_exact(sim.get_state(), ref.state(), "autotuned batched: ")
     t = sim.get_tuning()
-    assert all(c in (0, 1024, 2048, 4096) for c in t["tile_cap"])
+    assert all(c in (0, 1024, 2048, 4096, -1) for c in t["tile_cap"])
+
+
+@pytest.mark.parametrize("wide_max", [0, 1 << 30])
+def test_wide_and_narrow_pair_kernels_bit_exact(wide_max):
+    """One wave per particle (wide) and one lane per particle (narrow) give the
+    oracle's state bit for bit; the wide kernels also on a batched run."""
+    spec = dam_break_4k()
+    sim = gpu_for([spec])
+    sim.set_tuning(autotune=False, wide_max_particles=wide_max)
+    ref = oracle_for([spec])
+    sim.Update(DT, 200)
+    ref.step(DT, 200)
+    assert_bit_exact(sim.get_state(), ref.state(), f"wide_max {wide_max}: ")
+    st = sim.GetStats()
+    rs = ref.stats()
+    assert (st["minParticleNeighborCount"], st["maxParticleNeighborCount"]) == \
+        (rs["minParticleNeighborCount"], rs["maxParticleNeighborCount"])
+    specs = batched_scenes(seed=9, count=6, n_min=2048, n_max=4096)
+    bsim = gpu_for(specs)
+    bsim.set_tuning(autotune=False, wide_max_particles=wide_max)
+    bref = oracle_for(specs)
+    bsim.Update(DT, 40)
+    bref.step(DT, 40)
+    assert_bit_exact(bsim.get_state(), bref.state(), f"batched wide_max {wide_max}: ")
