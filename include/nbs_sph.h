@@ -47,7 +47,7 @@ typedef struct nbs_sim nbs_sim;
  * compile time: kSPHParticleCollisionRadius (sph.h:35-38) and
  * kSPHCollisionMargin (sph.h:53-54). */
 typedef struct nbs_params {
-  float kernel_height;       /* h */
+  float kernel_height;       /* h, in (0, 2^40] */
   float cell_size;           /* informational, as in SPHParameters; the grid uses nbs_domain */
   float particle_spacing;    /* emitter column spacing (demo4.cpp:258) */
   float rest_density;        /* rho0 */
@@ -234,9 +234,10 @@ nbs_status nbs_is_multithreading_supported(nbs_sim *sim, int32_t *supported);
 
 /* Device arithmetic self-test: out[6*i .. 6*i+5] = sqrtf(a[i]), 1.0f/a[i],
  * a[i]/b[i], the cell coordinate of a[i] for (half = b[i], size = 1,
- * n = 1<<20), and the pair kernels' correctly rounded square root and
- * reciprocal of a[i].  Used to show the device's FP32 arithmetic is the
- * correctly rounded IEEE arithmetic the reference's CPU build performs. */
+ * n = 1<<20), and the pair kernels' square root of a[i] and reciprocal of
+ * that root (1 for a[i] == 0), both correctly rounded for 0 <= a[i] < 2^80.
+ * Used to show the device's FP32 arithmetic is the correctly rounded IEEE
+ * arithmetic the reference's CPU build performs. */
 nbs_status nbs_selftest_math(const float *a, const float *b, uint64_t n, float *out);
 
 /* Device copy-bandwidth probe: copies `bytes` between two device buffers

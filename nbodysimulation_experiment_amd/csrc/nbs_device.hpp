@@ -3,6 +3,8 @@
 // candidate row ranges.
 #pragma once
 
+#include <climits>
+
 #include "nbs_kernels.hpp"
 
 namespace nbs {
@@ -44,34 +46,56 @@ __device__ __forceinline__ DevScene scene_of_slot(const Buffers &b, const DevSce
   return b.scenes[scene_of_key(b.sceneBase, b.nScenes, key)];
 }
 
-// Correctly rounded FP32 reciprocal and square root in fewer instructions than
-// the general IEEE lowering, for the operand ranges of the pair kernels.
+// Correctly rounded FP32 square root and reciprocal of the pair kernels'
+// distances, in fewer instructions than the general IEEE lowering and without
+// branches.  Operands: 0 <= r2 < h^2 <= 2^80 (kernel_height <= 2^40 is enforced
+// when parameters are set, and only pairs with r2 < h^2 reach these).
 //
-// rcp_rn: v_rcp_f32 (within 1 ulp) refined by one Newton step with FMA
-// (Markstein): e = 1 - x*y exactly, y' = y + e*y.  Used for x in [2^-60, 2^60];
-// other operands take the IEEE division.
-// sqrt_rn: v_sqrt_f32 followed by the residual test of the neighbouring floats
-// (s - ulp, s + ulp): x - s'*s computed with one FMA decides the rounding.  Used
-// for x == 0 and x in [2^-96, 2^100]; other operands take sqrtf.
-// Both are checked against the correctly rounded results for every mantissa of
-// several binades by the device arithmetic self-test.
-__device__ __forceinline__ float rcp_rn(float x) {
-  if (!(x >= 0x1p-60f && x <= 0x1p60f)) return 1.0f / x;
-  const float y = __builtin_amdgcn_rcpf(x);
-  const float e = __builtin_fmaf(-x, y, 1.0f);
-  return __builtin_fmaf(e, y, y);
-}
-
-__device__ __forceinline__ float sqrt_rn(float x) {
-  if (!((x >= 0x1p-96f && x <= 0x1p100f) || x == 0.0f)) return sqrtf(x);
+// v_sqrt_f32 and v_rcp_f32 are within 1 ulp for normal operands; the result is
+// then rounded correctly:
+//  - sqrt_near: of s and its neighbouring floats s - ulp, s + ulp, the residual
+//    x - s'*s (one FMA each) picks the correctly rounded one; exact for x == 0
+//    and x in [2^-96, 2^100];
+//  - rcp_near: one Newton step with FMA (Markstein), e = 1 - x*y exactly,
+//    y' = y + e*y; exact for x in [2^-60, 2^60].
+// r2 below 2^-96 (subnormals included) is scaled by 2^64 first.  Scaling by a
+// power of two is exact and commutes with correct rounding, so
+//   sqrt(r2) = sqrt(r2 * 2^64) * 2^-32  and  1/sqrt(r2) = (1/sqrt(r2 * 2^64)) * 2^32
+// with every intermediate inside the ranges above.  The device arithmetic
+// self-test compares both with the correctly rounded results for every
+// mantissa of the subnormals and of binades across the whole operand range.
+__device__ __forceinline__ float sqrt_near(float x) {
   float s = __builtin_amdgcn_sqrtf(x);
   const float dn = __uint_as_float(__float_as_uint(s) - 1u);
   const float up = __uint_as_float(__float_as_uint(s) + 1u);
   const float rdn = __builtin_fmaf(-dn, s, x);
   const float rup = __builtin_fmaf(-up, s, x);
-  if (rdn <= 0.0f) s = dn;
-  if (rup > 0.0f) s = up;
+  s = (rdn <= 0.0f) ? dn : s;
+  s = (rup > 0.0f) ? up : s;
   return s;
+}
+
+__device__ __forceinline__ float rcp_near(float x) {
+  const float y = __builtin_amdgcn_rcpf(x);
+  const float e = __builtin_fmaf(-x, y, 1.0f);
+  return __builtin_fmaf(e, y, y);
+}
+
+// r = sqrt(r2) correctly rounded.
+__device__ __forceinline__ float pair_sqrt(float r2) {
+  const bool tiny = r2 < 0x1p-96f;
+  const float s = sqrt_near(tiny ? r2 * 0x1p64f : r2);
+  return tiny ? s * 0x1p-32f : s;
+}
+
+// r = sqrt(r2) and inv = 1/r, each correctly rounded; inv = 1 for r == 0
+// (vecmath.h Vec2Normalize's zero-length case).
+__device__ __forceinline__ void pair_length(float r2, float &r, float &inv) {
+  const bool tiny = r2 < 0x1p-96f;
+  const float s = sqrt_near(tiny ? r2 * 0x1p64f : r2);
+  r = tiny ? s * 0x1p-32f : s;
+  const float y = rcp_near(s);
+  inv = (s == 0.0f) ? 1.0f : (tiny ? y * 0x1p32f : y);
 }
 
 // ---------------------------------------------------------------------------
@@ -115,9 +139,10 @@ __device__ __forceinline__ void solve_circle(const DevParams &p, float &x, float
 
 __device__ __forceinline__ float dot2(float ax, float ay, float bx, float by) { return ax * bx + ay * by; }
 
-__device__ __noinline__ void solve_segment(const DevParams &p, float &x, float &y, float ax, float ay, float bx,
-                                           float by) {
-  const float both = p.margin + p.R;                      // sph.h:545
+// The segment and polygon solvers are out of line (they are large and rarely
+// taken) and take and return the position by value, so the caller's x, y stay
+// in registers.
+__device__ __noinline__ float2 segment_push(float both, float x, float y, float ax, float ay, float bx, float by) {
   const float ex = bx - ax, ey = by - ay;                 // sph.h:548
   const float u = dot2(ex, ey, bx - x, by - y);           // sph.h:549
   const float v = dot2(ex, ey, x - ax, y - ay);           // sph.h:550
@@ -125,12 +150,12 @@ __device__ __noinline__ void solve_segment(const DevParams &p, float &x, float &
   if (v <= 0.0f) {                                        // sph.h:555-563
     clx = ax; cly = ay;
     const float dx = x - clx, dy = y - cly;
-    if (dot2(dx, dy, dx, dy) > both * both) return;
+    if (dot2(dx, dy, dx, dy) > both * both) return make_float2(x, y);
     normalize2(x - clx, y - cly, nx, ny);
   } else if (u <= 0.0f) {                                 // sph.h:564-572
     clx = bx; cly = by;
     const float dx = x - clx, dy = y - cly;
-    if (dot2(dx, dy, dx, dy) > both * both) return;
+    if (dot2(dx, dy, dx, dy) > both * both) return make_float2(x, y);
     normalize2(x - clx, y - cly, nx, ny);
   } else {                                                // sph.h:573-591
     const float den = dot2(ex, ey, ex, ey);
@@ -138,20 +163,25 @@ __device__ __noinline__ void solve_segment(const DevParams &p, float &x, float &
     clx = ((ax * u) + (bx * v)) * inv;
     cly = ((ay * u) + (by * v)) * inv;
     const float dx = x - clx, dy = y - cly;
-    if (dot2(dx, dy, dx, dy) > both * both) return;
+    if (dot2(dx, dy, dx, dy) > both * both) return make_float2(x, y);
     float px = -ey, py = ex;
     if (dot2(px, py, x - ax, y - ay) < 0.0f) { px = -px; py = -py; }
     normalize2(px, py, nx, ny);
   }
   const float dist = dot2(nx, ny, x - clx, y - cly);      // sph.h:593-594
   const float pen = both - dist;                          // sph.h:595
-  x = (nx * pen) + x;                                     // sph.h:596
-  y = (ny * pen) + y;
+  return make_float2((nx * pen) + x, (ny * pen) + y);     // sph.h:596
 }
 
-__device__ __noinline__ void solve_polygon(const DevParams &p, float &x, float &y, int nv, const float2 *verts) {
+__device__ __forceinline__ void solve_segment(const DevParams &p, float &x, float &y, float ax, float ay, float bx,
+                                              float by) {
+  const float2 r = segment_push(p.margin + p.R, x, y, ax, ay, bx, by);  // sph.h:545 both
+  x = r.x;
+  y = r.y;
+}
+
+__device__ __noinline__ float2 polygon_push(float radius, float x, float y, int nv, const float2 *verts) {
   // sph.h:600-672 FindMTVCirclePolygon, sph.h:674-681 SPHSolvePolygonCollision
-  const float radius = p.margin + p.R;
   int edge = 0;
   float nX = 0.0f, nY = 0.0f, sep = -3.402823466e+38f;
   for (int k = 0; k < nv; ++k) {
@@ -161,7 +191,7 @@ __device__ __noinline__ void solve_polygon(const DevParams &p, float &x, float &
     float cx, cy;
     normalize2(1.0f * ey, -1.0f * ex, cx, cy);  // vecmath.h:299-301 Vec2Cross(v, 1)
     const float s = dot2(cx, cy, x - a.x, y - a.y);
-    if (s > radius) return;
+    if (s > radius) return make_float2(x, y);
     if (s > sep) { nX = cx; nY = cy; sep = s; edge = k; }
   }
   const float2 v1 = verts[edge];
@@ -177,7 +207,7 @@ __device__ __noinline__ void solve_polygon(const DevParams &p, float &x, float &
       const float2 c = (u1 <= 0.0f) ? v1 : v2;
       // vecmath.h:292-296: the reference's "distance squared" is ((bx-ax)*(by-ay))^2
       const float f = (c.x - x) * (c.y - y);
-      if (f * f > radius * radius) return;
+      if (f * f > radius * radius) return make_float2(x, y);
       const float dx = x - c.x, dy = y - c.y;
       float ux, uy;
       normalize2(dx, dy, ux, uy);
@@ -188,24 +218,129 @@ __device__ __noinline__ void solve_polygon(const DevParams &p, float &x, float &
       const float fx = (1.0f - 0.5f) * v1.x + 0.5f * v2.x;
       const float fy = (1.0f - 0.5f) * v1.y + 0.5f * v2.y;
       const float s = dot2(x - fx, y - fy, nX, nY);
-      if (s > radius) return;
+      if (s > radius) return make_float2(x, y);
       const float pen = radius - s;
       mx = nX * pen; my = nY * pen;
     }
   }
-  x = x + mx;
-  y = y + my;
+  return make_float2(x + mx, y + my);
+}
+
+__device__ __forceinline__ void solve_polygon(const DevParams &p, float &x, float &y, int nv, const float2 *verts) {
+  const float2 r = polygon_push(p.margin + p.R, x, y, nv, verts);
+  x = r.x;
+  y = r.y;
+}
+
+// Body records do not change during a step: they are read through the
+// constant address space, so a wave-uniform index loads with scalar loads.
+__device__ __forceinline__ DevBody load_body(const DevBody *bodies, int k) {
+  const __attribute__((address_space(4))) DevBody &c = ((const __attribute__((address_space(4))) DevBody *)bodies)[k];
+  DevBody d;
+  d.type = c.type;
+  d.nverts = c.nverts;
+  d.vertOffset = c.vertOffset;
+  d.pad = c.pad;
+  d.a0 = c.a0, d.a1 = c.a1, d.a2 = c.a2, d.a3 = c.a3;
+  for (int q = 0; q < 4; ++q) d.cull[q] = c.cull[q];
+  return d;
 }
 
 __device__ __forceinline__ void collide(const Buffers &b, const DevParams &p, const DevScene &sc, float &x, float &y) {
   for (int k = sc.bodyBegin; k < sc.bodyEnd; ++k) {
-    const DevBody bd = b.bodies[k];
+    const DevBody bd = load_body(b.bodies, k);
     switch (bd.type) {
       case BODY_PLANE: solve_plane(p, x, y, bd.a0, bd.a1, bd.a2); break;
       case BODY_CIRCLE: solve_circle(p, x, y, bd.a0, bd.a1, bd.a2); break;
       case BODY_SEGMENT: solve_segment(p, x, y, bd.a0, bd.a1, bd.a2, bd.a3); break;
       case BODY_POLYGON: solve_polygon(p, x, y, bd.nverts, b.verts + bd.vertOffset); break;
       default: break;
+    }
+  }
+}
+
+// Bounding box of the live lanes' positions as float_order images, the same
+// in every lane.  All 64 lanes of the wave must be active.
+struct WaveBox {
+  int x0, y0, x1, y1;
+};
+
+__device__ __forceinline__ WaveBox wave_box(float x, float y, bool live) {
+  const int ox = float_order(__float_as_int(x)), oy = float_order(__float_as_int(y));
+  int x0 = live ? ox : INT_MAX, y0 = live ? oy : INT_MAX;
+  int x1 = live ? ox : INT_MIN, y1 = live ? oy : INT_MIN;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    x0 = min(x0, __shfl_xor(x0, o, 64));
+    y0 = min(y0, __shfl_xor(y0, o, 64));
+    x1 = max(x1, __shfl_xor(x1, o, 64));
+    y1 = max(y1, __shfl_xor(y1, o, 64));
+  }
+  return {__builtin_amdgcn_readfirstlane(x0), __builtin_amdgcn_readfirstlane(y0),
+          __builtin_amdgcn_readfirstlane(x1), __builtin_amdgcn_readfirstlane(y1)};
+}
+
+// Bodies among [c0, c0 + cnt) (cnt <= 64) that may move a position inside
+// `box`: bit t for body c0 + t.  A circle is excluded when the box lies
+// outside its cull box; every other body is included.
+__device__ __forceinline__ uint64_t body_candidates(const DevBody *bodies, int c0, int cnt, const WaveBox &box) {
+  const int t = threadIdx.x & 63;
+  bool may = false;
+  if (t < cnt) {
+    const DevBody &bd = bodies[c0 + t];
+    may = bd.type != BODY_CIRCLE ||
+          !(bd.cull[0] > box.x1 || bd.cull[2] < box.x0 || bd.cull[1] > box.y1 || bd.cull[3] < box.y0);
+  }
+  return __ballot(may);
+}
+
+// collide() for the live lanes of a wave, with the same results bit for bit.
+// When the live lanes share one body list, the wave walks it once: the lanes
+// test 64 bodies at a time against the wave's bounding box and only the
+// bodies that may touch it are applied, in body order.  A circle is skipped
+// when the box lies outside its cull box: every live position is then farther
+// than both * (1 + 2^-8) from the centre along one axis, so its FP32 test
+// d2 <= both^2 fails and the circle would leave it unchanged.  After a body
+// moves a live position the box and the remaining candidates are recomputed.
+// Lanes that are not live compute on whatever x, y hold and must not store
+// the result.  All 64 lanes of the wave must be active.
+template <bool MULTI>
+__device__ __forceinline__ void collide_wave(const Buffers &b, const DevParams &p, const DevScene &sc, float &x,
+                                             float &y, bool live) {
+  const uint64_t liveMask = __ballot(live);
+  if (!liveMask) return;
+  int k0 = sc.bodyBegin, k1 = sc.bodyEnd;
+  if (MULTI) {
+    const int src = __builtin_ctzll(liveMask);
+    k0 = __builtin_amdgcn_readfirstlane(__shfl(sc.bodyBegin, src, 64));
+    k1 = __builtin_amdgcn_readfirstlane(__shfl(sc.bodyEnd, src, 64));
+    if (__ballot(live && (sc.bodyBegin != k0 || sc.bodyEnd != k1))) {
+      if (live) collide(b, p, sc, x, y);
+      return;
+    }
+  }
+  if (k0 >= k1) return;
+  WaveBox box = wave_box(x, y, live);
+  for (int c0 = k0; c0 < k1; c0 += 64) {
+    const int cnt = min(64, k1 - c0);
+    uint64_t cand = body_candidates(b.bodies, c0, cnt, box);
+    while (cand) {
+      const int t = __builtin_amdgcn_readfirstlane(__builtin_ctzll(cand));
+      cand &= cand - 1u;
+      const DevBody bd = load_body(b.bodies, c0 + t);
+      const float px = x, py = y;
+      switch (bd.type) {
+        case BODY_PLANE: solve_plane(p, x, y, bd.a0, bd.a1, bd.a2); break;
+        case BODY_CIRCLE: solve_circle(p, x, y, bd.a0, bd.a1, bd.a2); break;
+        case BODY_SEGMENT: solve_segment(p, x, y, bd.a0, bd.a1, bd.a2, bd.a3); break;
+        case BODY_POLYGON: solve_polygon(p, x, y, bd.nverts, b.verts + bd.vertOffset); break;
+        default: break;
+      }
+      const bool moved = (__float_as_uint(x) != __float_as_uint(px)) || (__float_as_uint(y) != __float_as_uint(py));
+      if (__ballot(live && moved)) {
+        box = wave_box(x, y, live);
+        cand = body_candidates(b.bodies, c0, cnt, box) & ~((2ull << t) - 1u);  // bodies after t
+      }
     }
   }
 }

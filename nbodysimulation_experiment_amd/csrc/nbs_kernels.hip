@@ -143,9 +143,9 @@ __device__ __forceinline__ void viscosity_sum(const uint32_t *start, const DevSc
   for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t j, float Rx, float Ry, float r2) {
     const float2 vj = V[j];
     const float vjx = p.gdtx + vj.x, vjy = p.gdty + vj.y;
-    const float r = sqrt_rn(r2);                          // sph.h:503
+    float r, inv;
+    pair_length(r2, r, inv);                              // sph.h:503, 505 Vec2Normalize (length 0 -> 1)
     const float qq = r * p.invH;                          // sph.h:504
-    const float inv = (r == 0.0f) ? 1.0f : rcp_rn(r);     // sph.h:505 Vec2Normalize (length 0 -> 1)
     const float nX = Rx * inv, nY = Ry * inv;
     const float u = (vx - vjx) * nX + (vy - vjy) * nY;    // sph.h:506
     if (u > 0.0f) {
@@ -381,7 +381,7 @@ __device__ __forceinline__ uint32_t density_sum(const uint32_t *start, const Dev
                                                 const float2 *X, Idx idx, const DevParams &p, float2 xi, float &d0,
                                                 float &d1) {
   return for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t, float, float, float r2) {
-    const float r = sqrt_rn(r2);                          // sph.h:471
+    const float r = pair_sqrt(r2);                        // sph.h:471
     const float term = 1.0f - r * p.invH;                 // sph.h:472
     d0 = d0 + (term * term);                              // sph.h:473
     d1 = d1 + ((term * term) * term);                     // sph.h:474
@@ -446,8 +446,8 @@ __device__ __forceinline__ void delta_sum(const uint32_t *start, const DevScene 
                                           float &accx, float &accy) {
   for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t j, float Rx, float Ry, float r2) {
     const float2 pj = P[j];
-    const float r = sqrt_rn(r2);                          // sph.h:489
-    const float inv = (r == 0.0f) ? 1.0f : rcp_rn(r);     // sph.h:490 Vec2Normalize (length 0 -> 1)
+    float r, inv;
+    pair_length(r2, r, inv);                              // sph.h:489-490 Vec2Normalize (length 0 -> 1)
     const float nX = Rx * inv, nY = Ry * inv;
     const float term = 1.0f - r * p.invH;                 // sph.h:491
     const float tt = term * term;
@@ -471,26 +471,34 @@ __global__ __launch_bounds__(kBlock) void k_delta(Buffers b, DevParams p, DevSce
     stage(plan, b.x1, tx);
   }
   __syncthreads();
-  if (i >= n) return;
-  const uint32_t key = b.keys[i];
-  const DevScene sc = scene_of_slot<MULTI>(b, s0, key);
-  const uint32_t local = key - sc.cellBase;
-  const int cy = (int)(local / (uint32_t)sc.nx);
-  const int cx = (int)local - cy * sc.nx;
-  const float2 xi = b.x1[i];
-  const float2 pi = b.pres[i];
-  float accx = 0.0f, accy = 0.0f;
-  if (CAP > 0 && plan.use)
-    delta_sum(b.startNext, sc, cx, cy, tx, b.pres, TileIndex(plan), p, xi, pi, accx, accy);
-  else
-    delta_sum(b.startNext, sc, cx, cy, b.x1, b.pres, SlotIndex(), p, xi, pi, accx, accy);
-  if (b.dxOut) b.dxOut[i] = make_float2(accx, accy);
-  float x = accx + xi.x;                                  // demo4.cpp:253
-  float y = accy + xi.y;
-  collide(b, p, sc, x, y);                                // demo4.cpp:411-444
-  const float2 pv = b.prev[i];
-  b.x0[i] = make_float2(x, y);
-  b.v[i] = make_float2((x - pv.x) * p.invDt, (y - pv.y) * p.invDt);  // demo4.cpp:449
+  // Lanes past n stay active (collide_wave reduces over the whole wave) but
+  // read and write nothing.
+  const bool live = i < n;
+  DevScene sc = s0;
+  float x = 0.0f, y = 0.0f;
+  if (live) {
+    const uint32_t key = b.keys[i];
+    sc = scene_of_slot<MULTI>(b, s0, key);
+    const uint32_t local = key - sc.cellBase;
+    const int cy = (int)(local / (uint32_t)sc.nx);
+    const int cx = (int)local - cy * sc.nx;
+    const float2 xi = b.x1[i];
+    const float2 pi = b.pres[i];
+    float accx = 0.0f, accy = 0.0f;
+    if (CAP > 0 && plan.use)
+      delta_sum(b.startNext, sc, cx, cy, tx, b.pres, TileIndex(plan), p, xi, pi, accx, accy);
+    else
+      delta_sum(b.startNext, sc, cx, cy, b.x1, b.pres, SlotIndex(), p, xi, pi, accx, accy);
+    if (b.dxOut) b.dxOut[i] = make_float2(accx, accy);
+    x = accx + xi.x;                                      // demo4.cpp:253
+    y = accy + xi.y;
+  }
+  collide_wave<MULTI>(b, p, sc, x, y, live);              // demo4.cpp:411-444
+  if (live) {
+    const float2 pv = b.prev[i];
+    b.x0[i] = make_float2(x, y);
+    b.v[i] = make_float2((x - pv.x) * p.invDt, (y - pv.y) * p.invDt);  // demo4.cpp:449
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -536,9 +544,9 @@ __global__ __launch_bounds__(kBlock) void k_visc_predict_wide(Buffers b, DevPara
         [&](uint32_t j, float Rx, float Ry, float r2, float *t) {
           const float2 vj = V[j];
           const float vjx = p.gdtx + vj.x, vjy = p.gdty + vj.y;
-          const float r = sqrt_rn(r2);                          // sph.h:503
+          float r, inv;
+          pair_length(r2, r, inv);                              // sph.h:503, 505
           const float qq = r * p.invH;                          // sph.h:504
-          const float inv = (r == 0.0f) ? 1.0f : rcp_rn(r);     // sph.h:505
           const float nX = Rx * inv, nY = Ry * inv;
           const float u = (vx - vjx) * nX + (vy - vjy) * nY;    // sph.h:506
           if (!(u > 0.0f)) return false;
@@ -592,7 +600,7 @@ __global__ __launch_bounds__(kBlock) void k_density_wide(Buffers b, DevParams p,
   const uint32_t cand = for_each_pair_wide<2>(
       b.startNext, sc, cx, cy, b.x1, SlotIndex(), xi, p.h2,
       [&](uint32_t, float, float, float r2, float *t) {
-        const float r = sqrt_rn(r2);                            // sph.h:471
+        const float r = pair_sqrt(r2);                          // sph.h:471
         const float term = 1.0f - r * p.invH;                   // sph.h:472
         t[0] = term * term;
         t[1] = (term * term) * term;
@@ -628,8 +636,8 @@ __global__ __launch_bounds__(kBlock) void k_delta_wide(Buffers b, DevParams p, D
       b.startNext, sc, cx, cy, b.x1, SlotIndex(), xi, p.h2,
       [&](uint32_t j, float Rx, float Ry, float r2, float *t) {
         const float2 pj = P[j];
-        const float r = sqrt_rn(r2);                            // sph.h:489
-        const float inv = (r == 0.0f) ? 1.0f : rcp_rn(r);       // sph.h:490
+        float r, inv;
+        pair_length(r2, r, inv);                                // sph.h:489-490
         const float nX = Rx * inv, nY = Ry * inv;
         const float term = 1.0f - r * p.invH;                   // sph.h:491
         const float tt = term * term;
@@ -647,11 +655,12 @@ __global__ __launch_bounds__(kBlock) void k_delta_wide(Buffers b, DevParams p, D
         accx = accx - v[2];                                     // demo4.cpp:250 (j's visit)
         accy = accy - v[3];
       });
+  // Every lane holds the same sums: the wave walks the bodies once for lane 0.
+  float x = accx + xi.x;                                        // demo4.cpp:253
+  float y = accy + xi.y;
+  collide_wave<MULTI>(b, p, sc, x, y, (threadIdx.x & 63) == 0);  // demo4.cpp:411-444
   if ((threadIdx.x & 63) == 0) {
     if (b.dxOut) b.dxOut[i] = make_float2(accx, accy);
-    float x = accx + xi.x;                                      // demo4.cpp:253
-    float y = accy + xi.y;
-    collide(b, p, sc, x, y);                                    // demo4.cpp:411-444
     const float2 pv = b.prev[i];
     b.x0[i] = make_float2(x, y);
     b.v[i] = make_float2((x - pv.x) * p.invDt, (y - pv.y) * p.invDt);  // demo4.cpp:449
@@ -791,8 +800,10 @@ __global__ __launch_bounds__(kBlock) void k_selftest_math(const float *a, const 
   out[6 * i + 1] = 1.0f / x;
   out[6 * i + 2] = x / y;
   out[6 * i + 3] = (float)cell_coord(x, y, 1.0f, 1 << 20);
-  out[6 * i + 4] = sqrt_rn(x);
-  out[6 * i + 5] = rcp_rn(x);
+  float r, inv;
+  pair_length(x, r, inv);
+  out[6 * i + 4] = pair_sqrt(x);
+  out[6 * i + 5] = inv;
 }
 
 // Plain device copy (the bandwidth probe and counter calibration).

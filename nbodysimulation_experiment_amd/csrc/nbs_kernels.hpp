@@ -39,7 +39,14 @@ struct DevBody {
   int vertOffset;
   int pad;
   float a0, a1, a2, a3;
+  // Circles: float_order images of a box outside which no position can pass
+  // the circle's collision test (x0, y0, x1, y1); full range for other bodies.
+  int cull[4];
 };
+
+// Order-preserving integer image of a float: for non-NaN a, b,
+// a < b implies float_order(a) < float_order(b) (and -0 maps below +0).
+__host__ __device__ inline int float_order(int bits) { return bits ^ ((bits >> 31) & 0x7fffffff); }
 
 // Device storage.  Particle arrays are indexed by slot: slots [0, nBinned)
 // hold the particles of the last binning sorted by (cell key, particle index);

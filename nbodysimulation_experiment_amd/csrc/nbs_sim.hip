@@ -114,8 +114,9 @@ class Simulation {
 
   // ---- parameters -------------------------------------------------------
   static void validateParams(const nbs_params &p) {
-    if (!(p.kernel_height > 0.0f) || !std::isfinite(p.kernel_height))
-      throw Error(NBS_ERR_INVALID_ARGUMENT, "kernel_height must be positive and finite");
+    // The pair kernels' square root and reciprocal take r^2 < h^2 <= 2^80.
+    if (!(p.kernel_height > 0.0f && p.kernel_height <= 0x1p40f))
+      throw Error(NBS_ERR_INVALID_ARGUMENT, "kernel_height must be in (0, 2^40]");
     const float v[] = {p.rest_density, p.stiffness, p.near_stiffness, p.linear_viscosity, p.quadratic_viscosity,
                        p.collision_radius, p.collision_margin, p.particle_spacing, p.cell_size};
     for (float x : v)
@@ -124,6 +125,7 @@ class Simulation {
   void setParams(const nbs_params &p) {
     validateParams(p);
     params_ = p;
+    bodiesDirty_ = true;  // circle cull boxes depend on the collision radius
   }
   const nbs_params &params() const { return params_; }
   void setGravity(float gx, float gy) { gravity_[0] = gx, gravity_[1] = gy; }
@@ -764,6 +766,39 @@ class Simulation {
     buf_.scenes = devScenes_.p; buf_.sceneBase = devSceneBase_.p; buf_.nScenes = (int)scenes_.size();
     buf_.bodies = devBodies_.p; buf_.verts = devVerts_.p;
   }
+  // Cull box of a circle (collide_wave): positions x < cx - reach or
+  // x > cx + reach (likewise y), reach = both * (1 + 2^-8) + 2^-40 with
+  // both = r + R as the kernel forms it, fail the kernel's FP32 test
+  // d2 <= both * both: |fl(x - cx)| >= reach * (1 - 2^-24), so d2 >= reach^2 *
+  // (1 - 2^-24)^3 > both^2 * (1 + 2^-24) >= fl(both * both).  The bounds are
+  // rounded outwards.  Other bodies get the full range (never skipped).
+  void setCullBox(DevBody &d) const {
+    d.cull[0] = d.cull[1] = INT_MIN;
+    d.cull[2] = d.cull[3] = INT_MAX;
+    if (d.type != BODY_CIRCLE) return;
+    const float both = d.a2 + params_.collision_radius;  // sph.h:527
+    if (!std::isfinite(both)) return;
+    const double reach = (double)both * (1.0 + 0x1p-8) + 0x1p-40;
+    auto down = [](double v) {
+      float f = (float)v;
+      if ((double)f > v) f = std::nextafter(f, -INFINITY);
+      return f;
+    };
+    auto up = [](double v) {
+      float f = (float)v;
+      if ((double)f < v) f = std::nextafter(f, INFINITY);
+      return f;
+    };
+    auto order = [](float f) {
+      int bits;
+      std::memcpy(&bits, &f, 4);
+      return float_order(bits);
+    };
+    d.cull[0] = order(down((double)d.a0 - reach));
+    d.cull[1] = order(down((double)d.a1 - reach));
+    d.cull[2] = order(up((double)d.a0 + reach));
+    d.cull[3] = order(up((double)d.a1 + reach));
+  }
   // Uploads scene descriptors and the body table when they changed.
   void syncBodies() {
     if (!bodiesDirty_ && !scenesDirty_) return;
@@ -777,6 +812,7 @@ class Simulation {
         DevBody d{};
         d.type = hb.type;
         d.a0 = hb.a[0]; d.a1 = hb.a[1]; d.a2 = hb.a[2]; d.a3 = hb.a[3];
+        setCullBox(d);
         if (hb.type == BODY_POLYGON) {
           d.nverts = (int)(hb.verts.size() / 2);
           d.vertOffset = (int)verts.size();

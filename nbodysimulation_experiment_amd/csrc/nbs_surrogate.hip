@@ -205,7 +205,7 @@ __device__ __forceinline__ void pair_words(uint32_t w[8], const PairSrc &src, co
   const float2 pj = src.pv[t];
   const float2 dj = src.d[t];
   const float vjx = (xj.x - pj.x) * p.invDt, vjy = (xj.y - pj.y) * p.invDt;
-  const float r = sqrt_rn(r2);
+  const float r = pair_sqrt(r2);
   const float qq = 1.0f - r * p.invH;
   w[0] = pack2(Rx * p.invH, Ry * p.invH);
   w[1] = pack2((vjx - q.vix) * norm[0], (vjy - q.viy) * norm[0]);
@@ -387,18 +387,22 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
     float o = prm[kOffB4 + g];
     for (int k = 0; k < kHid; ++k) o = o + prm[kOffW4 + g * kHid + k] * gbuf[c][k];
     const float oy = __shfl_down(o, 32, 64);
-    if (g == 0 && i < n) {
-      if (m.out) {
-        m.out[2 * (size_t)i] = o;
-        m.out[2 * (size_t)i + 1] = oy;
-      }
-      if (mode & kFwdState) {
-        const uint32_t key = b.keys[i];
-        const DevScene sc = scene_of_slot<MULTI>(b, s0, key);
+    const bool head = g == 0 && i < n;
+    if (head && m.out) {
+      m.out[2 * (size_t)i] = o;
+      m.out[2 * (size_t)i + 1] = oy;
+    }
+    if (mode & kFwdState) {
+      DevScene sc = s0;
+      float x = 0.0f, y = 0.0f;
+      if (head) {
+        sc = scene_of_slot<MULTI>(b, s0, b.keys[i]);
         const float2 xi = b.x1[i];
-        float x = xi.x + o * norm[5];
-        float y = xi.y + oy * norm[5];
-        collide(b, p, sc, x, y);                                               // demo4.cpp:411-444
+        x = xi.x + o * norm[5];
+        y = xi.y + oy * norm[5];
+      }
+      collide_wave<MULTI>(b, p, sc, x, y, head);                               // demo4.cpp:411-444
+      if (head) {
         const float2 pv = b.prev[i];
         b.x0[i] = make_float2(x, y);
         b.v[i] = make_float2((x - pv.x) * p.invDt, (y - pv.y) * p.invDt);      // demo4.cpp:446-450

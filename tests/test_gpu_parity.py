@@ -23,6 +23,14 @@ def _small_scene(seed=3, circles=True):
                      circles=[(3.0, -5.0, 1.5)] if circles else [])
 
 
+def _pair_inv(a):
+    """1/sqrt(a) rounded after each operation, and 1 for a == 0."""
+    r = np.sqrt(a)
+    with np.errstate(divide="ignore"):
+        inv = np.float32(1) / r
+    return np.where(r == 0, np.float32(1), inv).astype(np.float32)
+
+
 def test_device_sqrt_and_divide_are_correctly_rounded():
     from nbodysimulation_experiment_amd.sim import selftest_math
     rng = np.random.default_rng(0)
@@ -34,41 +42,35 @@ def test_device_sqrt_and_divide_are_correctly_rounded():
         np.testing.assert_array_equal(out[:, 0].view(np.uint32), np.sqrt(a).view(np.uint32))
         np.testing.assert_array_equal(out[:, 1].view(np.uint32), (np.float32(1) / a).view(np.uint32))
         np.testing.assert_array_equal(out[:, 2].view(np.uint32), (a / b).view(np.uint32))
-        np.testing.assert_array_equal(out[:, 4].view(np.uint32), np.sqrt(a).view(np.uint32))
-        np.testing.assert_array_equal(out[:, 5].view(np.uint32), (np.float32(1) / a).view(np.uint32))
+    ok = a < np.float32(2.0 ** 80)  # the pair kernels' operand range
+    np.testing.assert_array_equal(out[ok, 4].view(np.uint32), np.sqrt(a[ok]).view(np.uint32))
+    np.testing.assert_array_equal(out[ok, 5].view(np.uint32), _pair_inv(a[ok]).view(np.uint32))
 
 
 def _binade(e):
     mant = np.arange(1 << 23, dtype=np.uint32)
+    if e < -126:  # all subnormals (and zero)
+        return mant.view(np.float32)
     return ((np.uint32(e + 127) << np.uint32(23)) | mant).view(np.float32)
 
 
 def test_pair_kernel_rcp_and_sqrt_exhaustive_binades():
-    """Every mantissa of the binades spanning the fast paths' ranges: the pair
-    kernels' reciprocal and square root equal the correctly rounded results."""
+    """Every mantissa of the subnormals and of binades spanning the pair
+    kernels' operand range [0, 2^80), on both sides of the 2^-96 scaling
+    threshold: the pair square root and the reciprocal of that root equal the
+    correctly rounded results."""
     from nbodysimulation_experiment_amd.sim import selftest_math
-    one = None
-    for e in (-60, -47, -20, -1, 0, 1, 7, 33, 59, 60):
-        a = _binade(e)
-        one = np.ones_like(a) if one is None else one
-        out = selftest_math(a, one)
-        with np.errstate(over="ignore"):
-            np.testing.assert_array_equal(out[:, 5].view(np.uint32), (np.float32(1) / a).view(np.uint32),
-                                          err_msg=f"rcp binade 2^{e}")
-            np.testing.assert_array_equal(out[:, 1].view(np.uint32), (np.float32(1) / a).view(np.uint32))
-    for e in (-96, -95, -41, -2, -1, 0, 1, 40, 99, 100):
+    one = np.ones(1 << 23, np.float32)
+    for e in (-127, -126, -125, -100, -97, -96, -95, -80, -61, -60, -47, -20, -2, -1, 0, 1, 7, 33, 59, 79):
         a = _binade(e)
         out = selftest_math(a, one)
         np.testing.assert_array_equal(out[:, 4].view(np.uint32), np.sqrt(a).view(np.uint32),
                                       err_msg=f"sqrt binade 2^{e}")
+        np.testing.assert_array_equal(out[:, 5].view(np.uint32), _pair_inv(a).view(np.uint32),
+                                      err_msg=f"inverse length binade 2^{e}")
         np.testing.assert_array_equal(out[:, 0].view(np.uint32), np.sqrt(a).view(np.uint32))
-    # fallback ranges and special values
-    sp = np.array([0.0, -0.0, 1e-45, 1e-40, 2.0 ** -97, 2.0 ** -61, 2.0 ** 61, 2.0 ** 101, 3.4e38, np.inf],
-                  np.float32)
-    out = selftest_math(sp, np.ones_like(sp))
-    with np.errstate(divide="ignore", over="ignore"):
-        np.testing.assert_array_equal(out[:, 4].view(np.uint32), np.sqrt(sp).view(np.uint32))
-        np.testing.assert_array_equal(out[:, 5].view(np.uint32), (np.float32(1) / sp).view(np.uint32))
+        with np.errstate(divide="ignore", over="ignore"):
+            np.testing.assert_array_equal(out[:, 1].view(np.uint32), (np.float32(1) / a).view(np.uint32))
 
 
 def test_device_cell_coordinate_matches_oracle():
