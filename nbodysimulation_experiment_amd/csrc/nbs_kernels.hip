@@ -27,10 +27,11 @@ namespace nbs {
 // cell (cx, cy) whose r2 < h2 (sph.h:470, 488, 502), in candidate order, reading
 // positions from X at idx(d, j) (the LDS tile or the slot array); returns the
 // number of candidates (the reference's neighbour count).  Each row range is
-// taken in chunks of 64: a cheap pass marks the in-kernel candidates in a
-// 64-bit mask, then the pair work runs only for the marked ones, lowest first,
-// so the accumulation order is the candidate order and the expensive work is
-// not issued for candidates outside the kernel radius.
+// taken in chunks of 32: a cheap pass marks the in-kernel candidates in a
+// 32-bit mask (the chunk index t is the same in every lane, so its bit is a
+// scalar), then the pair work runs only for the marked ones, lowest first, so
+// the accumulation order is the candidate order and the expensive work is not
+// issued for candidates outside the kernel radius.
 template <typename Idx, typename F>
 __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const DevScene &sc, int cx, int cy,
                                                   const float2 *X, Idx idx, float2 xi, float h2, F &&f) {
@@ -38,19 +39,19 @@ __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const D
   for (int d = 0; d < 3; ++d) {
     const RowRange rr = row_range(start, sc, cx, cy, d - 1);
     cand += rr.j1 - rr.j0;
-    for (uint32_t base = rr.j0; base < rr.j1; base += 64u) {
-      const uint32_t cnt = min(64u, rr.j1 - base);
+    for (uint32_t base = rr.j0; base < rr.j1; base += 32u) {
+      const uint32_t cnt = min(32u, rr.j1 - base);
       const uint32_t q0 = idx(d, base);
-      uint64_t mask = 0;
+      uint32_t mask = 0;
 #pragma unroll 4
       for (uint32_t t = 0; t < cnt; ++t) {
         const float2 xj = X[q0 + t];
         const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;
         const float r2 = Rx * Rx + Ry * Ry;
-        mask |= (uint64_t)(r2 < h2) << t;
+        mask |= (r2 < h2) ? (1u << t) : 0u;
       }
       while (mask) {
-        const uint32_t t = (uint32_t)__builtin_ctzll(mask);
+        const uint32_t t = (uint32_t)__builtin_ctz(mask);
         mask &= mask - 1u;
         const float2 xj = X[q0 + t];
         const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;
