@@ -81,21 +81,31 @@ __device__ __forceinline__ float rcp_near(float x) {
   return __builtin_fmaf(e, y, y);
 }
 
-// r = sqrt(r2) correctly rounded.
+// r = sqrt(r2) correctly rounded.  The scaled form runs only in waves where
+// some active lane has r2 < 2^-96 (a uniform branch).
 __device__ __forceinline__ float pair_sqrt(float r2) {
-  const bool tiny = r2 < 0x1p-96f;
-  const float s = sqrt_near(tiny ? r2 * 0x1p64f : r2);
-  return tiny ? s * 0x1p-32f : s;
+  if (__builtin_expect(__ballot(r2 < 0x1p-96f) != 0, 0)) {
+    const bool tiny = r2 < 0x1p-96f;
+    const float s = sqrt_near(tiny ? r2 * 0x1p64f : r2);
+    return tiny ? s * 0x1p-32f : s;
+  }
+  return sqrt_near(r2);
 }
 
 // r = sqrt(r2) and inv = 1/r, each correctly rounded; inv = 1 for r == 0
-// (vecmath.h Vec2Normalize's zero-length case).
+// (vecmath.h Vec2Normalize's zero-length case).  r2 >= 2^-96 gives r >= 2^-48,
+// so outside the scaled form r is never 0.
 __device__ __forceinline__ void pair_length(float r2, float &r, float &inv) {
-  const bool tiny = r2 < 0x1p-96f;
-  const float s = sqrt_near(tiny ? r2 * 0x1p64f : r2);
-  r = tiny ? s * 0x1p-32f : s;
-  const float y = rcp_near(s);
-  inv = (s == 0.0f) ? 1.0f : (tiny ? y * 0x1p32f : y);
+  if (__builtin_expect(__ballot(r2 < 0x1p-96f) != 0, 0)) {
+    const bool tiny = r2 < 0x1p-96f;
+    const float s = sqrt_near(tiny ? r2 * 0x1p64f : r2);
+    r = tiny ? s * 0x1p-32f : s;
+    const float y = rcp_near(s);
+    inv = (s == 0.0f) ? 1.0f : (tiny ? y * 0x1p32f : y);
+    return;
+  }
+  r = sqrt_near(r2);
+  inv = rcp_near(r);
 }
 
 // ---------------------------------------------------------------------------
