@@ -13,15 +13,6 @@ namespace nbs {
 // Helpers
 // ---------------------------------------------------------------------------
 
-// Workgroup index with consecutive tiles kept on one XCD.  The dispatcher
-// places workgroup b on XCD (b mod 8); this remap gives XCD x a contiguous
-// range of tiles so neighbouring cells share that XCD's L2.
-__device__ __forceinline__ unsigned xcd_block() {
-  const unsigned b = blockIdx.x, nb = gridDim.x;
-  const unsigned q = nb >> 3, r = nb & 7u, x = b & 7u, loc = b >> 3;
-  return (x < r) ? x * (q + 1u) + loc : r * (q + 1u) + (x - r) * q + loc;
-}
-
 // sph.h:450-463 per axis.  Out-of-range values are clamped in floating point so
 // NaN and values beyond the int range are defined (NaN -> 0).
 __device__ __forceinline__ int cell_coord(float p, float half, float cs, int n) {

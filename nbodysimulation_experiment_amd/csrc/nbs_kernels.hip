@@ -166,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_visc_predict(Buffers b, DevParams p,
                                                          int lists) {
   __shared__ float2 tx[CAP > 0 ? CAP : 1];
   __shared__ TilePlan plan;
-  const int i0 = (int)(xcd_block() * kBlock);
+  const int i0 = (int)(blockIdx.x * kBlock);
   const int i = i0 + (int)threadIdx.x;
   // Viscosity runs for slots of the previous binning when lists exist.
   const bool tileVisc = lists && i0 < nBinned;
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(Buffers b, uint32_t cells
 // 3. bucket by cell, order each cell by particle index, move the state
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_scatter(Buffers b, int n) {
-  const int s = (int)(xcd_block() * kBlock + threadIdx.x);
+  const int s = (int)(blockIdx.x * kBlock + threadIdx.x);
   if (s >= n) return;
   const uint32_t key = b.keyp[s];
   const uint32_t pos = b.startNext[key] + b.fill[s];
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Buffers b, int n) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
-  const int p = (int)(xcd_block() * kBlock + threadIdx.x);
+  const int p = (int)(blockIdx.x * kBlock + threadIdx.x);
   if (p >= n) return;
   const uint32_t s = b.tmpSlot[p];
   const uint32_t myId = b.tmpId[p];
@@ -405,7 +405,7 @@ template <bool MULTI, int CAP>
 __global__ __launch_bounds__(kBlock) void k_density(Buffers b, DevParams p, DevScene s0, int n) {
   __shared__ float2 tx[CAP > 0 ? CAP : 1];
   __shared__ TilePlan plan;
-  const int i0 = (int)(xcd_block() * kBlock);
+  const int i0 = (int)(blockIdx.x * kBlock);
   const int i = i0 + (int)threadIdx.x;
   plan_new<MULTI, CAP>(b, s0, i0, n, plan);
   if (CAP > 0 && plan.use) stage(plan, b.x1, tx);
@@ -465,7 +465,7 @@ template <bool MULTI, int CAP>
 __global__ __launch_bounds__(kBlock) void k_delta(Buffers b, DevParams p, DevScene s0, int n) {
   __shared__ float2 tx[CAP > 0 ? CAP : 1];
   __shared__ TilePlan plan;
-  const int i0 = (int)(xcd_block() * kBlock);
+  const int i0 = (int)(blockIdx.x * kBlock);
   const int i = i0 + (int)threadIdx.x;
   plan_new<MULTI, CAP>(b, s0, i0, n, plan);
   if (CAP > 0 && plan.use) {
