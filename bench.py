@@ -181,6 +181,7 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="also run the library's copy kernel on MiB of data (3 times) for counter calibration")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-relaxation leg")
+    ap.add_argument("--no-graph-leg", action="store_true", help="skip the HIP-graph replay leg")
     ap.add_argument("--train-scenes", type=int, default=200)
     ap.add_argument("--train-warm", type=int, default=100)
     ap.add_argument("--train-steps", type=int, default=500)
@@ -242,6 +243,21 @@ def main():
     end_state = sim.get_state() if not args.no_cpu_baseline else None
     sim.close()
     log(f"{value:.4g} particle-steps/s, {1e3 * elapsed / args.steps:.4f} ms/step; kernels {kt}")
+
+    graph_leg = None
+    if not args.graph and not args.no_graph_leg:
+        # The same K steps of a fresh instance with steady-state steps replayed
+        # as HIP graphs (no per-kernel events; tuning steps still run eagerly).
+        gsim = make_sim()
+        gsim.set_options(hip_graph=True)
+        log(f"timing {args.steps} steps with HIP-graph replay")
+        t0 = time.perf_counter()
+        gsim.Update(DT, args.steps)
+        g_elapsed = time.perf_counter() - t0
+        gsim.close()
+        graph_leg = {"value": round(n * args.steps / g_elapsed, 1), "unit": "particle-steps/s",
+                     "ms_per_step": round(1e3 * g_elapsed / args.steps, 5)}
+        log(f"graph replay {n * args.steps / g_elapsed:.4g} particle-steps/s")
 
     roofline = None
     if not args.graph:
@@ -326,7 +342,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": desc, "particles": n, "dt": DT, "method": "plain", "hip_graph": args.graph,
                    "tuning": tuning},
-        "roofline": roofline, "cpu_baseline": cpu, "surrogate": surrogate,
+        "roofline": roofline, "cpu_baseline": cpu, "hip_graph_replay": graph_leg, "surrogate": surrogate,
         "stats": {k: v for k, v in stats.items() if k != "time"},
         "diagnostics_fp64": {k: diag[k] for k in ("mean_density", "max_density", "mean_abs_density_error",
                                                   "kinetic_energy", "non_finite_count")},
