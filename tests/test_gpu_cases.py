@@ -218,3 +218,18 @@ def test_many_circles_and_mixed_body_lists_bit_exact(caps):
         sim.Update(DT, 160)
         ref.step(DT, 160)
         assert_bit_exact(sim.get_state(), ref.state(), f"{len(specs)} scene(s), caps {caps}: ")
+
+
+@pytest.mark.parametrize("caps", [(1024, 1024, 1024), (0, 0, 2048), (1024, -1, 1024)])
+def test_dense_cells_bit_exact(caps):
+    """Particles at 0.12h spacing put about 70 in a cell: rows of 200+
+    candidates walked in 32-candidate chunks (more than 16 per particle), in
+    LDS-tile, global-memory and wide kernels, against the oracle bit for bit."""
+    pos = volume_positions((-2.0, -2.0), 34, 34, 0.12, 1.0, seed=31)
+    spec = SceneSpec(20.0, 20.0, 1.0, pos, planes=wall_planes(20.0, 20.0))
+    sim, ref = gpu_for([spec]), oracle_for([spec])
+    sim.set_tuning(autotune=False, tile_cap=caps, wide_max_particles=0)
+    sim.Update(DT, 12)
+    ref.step(DT, 12)
+    assert_bit_exact(sim.get_state(), ref.state(), f"dense, caps {caps}: ")
+    assert sim.GetStats()["maxParticleNeighborCount"] > 16 * 32
