@@ -579,44 +579,49 @@ __global__ __launch_bounds__(kBlock) void k_visc_predict_wide(Buffers b, DevPara
 
 // (min, max) candidate-count pairs of the one-lane-per-particle workgroups,
 // reset before the wide density kernel folds its counts in.
-__global__ __launch_bounds__(kBlock) void k_reset_minmax(uint32_t *mm, int pairs) {
-  const int k = (int)(blockIdx.x * kBlock + threadIdx.x);
-  if (k < pairs) {
-    mm[2 * k] = 0xFFFFFFFFu;
-    mm[2 * k + 1] = 0u;
-  }
-}
-
+// Candidate-count statistics: one min/max pair per workgroup (kWidePerBlock
+// particles), slot blockIdx.x.
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_density_wide(Buffers b, DevParams p, DevScene s0, int n) {
-  const int i = (int)(blockIdx.x * kWidePerBlock + (threadIdx.x >> 6));
-  if (i >= n) return;  // whole wave
-  const uint32_t key = b.keys[i];
-  const DevScene sc = scene_of_slot<MULTI>(b, s0, key);
-  const uint32_t local = key - sc.cellBase;
-  const int cy = (int)(local / (uint32_t)sc.nx);
-  const int cx = (int)local - cy * sc.nx;
-  const float2 xi = b.x1[i];
-  float d0 = 0.0f, d1 = 0.0f;
-  const uint32_t cand = for_each_pair_wide<2>(
-      b.startNext, sc, cx, cy, b.x1, SlotIndex(), xi, p.h2,
-      [&](uint32_t, float, float, float r2, float *t) {
-        const float r = pair_sqrt(r2);                          // sph.h:471
-        const float term = 1.0f - r * p.invH;                   // sph.h:472
-        t[0] = term * term;
-        t[1] = (term * term) * term;
-        return true;
-      },
-      [&](const float *v) {
-        d0 = d0 + v[0];                                         // sph.h:473
-        d1 = d1 + v[1];                                         // sph.h:474
-      });
-  if ((threadIdx.x & 63) == 0) {
-    b.dens[i] = make_float2(d0, d1);
-    b.pres[i] = make_float2(p.k * (d0 - p.rho0), p.kn * d1);   // sph.h:478-481
-    const int w = i / kBlock;                                   // statistics slot of the narrow layout
-    atomicMin(&b.candStats[2 * w], cand);
-    atomicMax(&b.candStats[2 * w + 1], cand);
+  __shared__ uint32_t shCand[kWidePerBlock];
+  const int w = (int)(threadIdx.x >> 6);
+  const int i = (int)(blockIdx.x * kWidePerBlock + w);
+  if (i < n) {  // whole wave
+    const uint32_t key = b.keys[i];
+    const DevScene sc = scene_of_slot<MULTI>(b, s0, key);
+    const uint32_t local = key - sc.cellBase;
+    const int cy = (int)(local / (uint32_t)sc.nx);
+    const int cx = (int)local - cy * sc.nx;
+    const float2 xi = b.x1[i];
+    float d0 = 0.0f, d1 = 0.0f;
+    const uint32_t cand = for_each_pair_wide<2>(
+        b.startNext, sc, cx, cy, b.x1, SlotIndex(), xi, p.h2,
+        [&](uint32_t, float, float, float r2, float *t) {
+          const float r = pair_sqrt(r2);                        // sph.h:471
+          const float term = 1.0f - r * p.invH;                 // sph.h:472
+          t[0] = term * term;
+          t[1] = (term * term) * term;
+          return true;
+        },
+        [&](const float *v) {
+          d0 = d0 + v[0];                                       // sph.h:473
+          d1 = d1 + v[1];                                       // sph.h:474
+        });
+    if ((threadIdx.x & 63) == 0) {
+      b.dens[i] = make_float2(d0, d1);
+      b.pres[i] = make_float2(p.k * (d0 - p.rho0), p.kn * d1); // sph.h:478-481
+      shCand[w] = cand;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+    for (int k = 0; k < kWidePerBlock && (int)(blockIdx.x * kWidePerBlock) + k < n; ++k) {
+      mn = min(mn, shCand[k]);
+      mx = max(mx, shCand[k]);
+    }
+    b.candStats[2 * blockIdx.x] = mn;
+    b.candStats[2 * blockIdx.x + 1] = mx;
   }
 }
 
@@ -834,6 +839,8 @@ static void visc_cap(const Buffers &b, const DevParams &p, const DevScene &s0, i
 
 static unsigned wide_grid(int n) { return (unsigned)((n + kWidePerBlock - 1) / kWidePerBlock); }
 
+uint32_t density_stat_slots(int n, int cap) { return cap == kWideCap ? wide_grid(n) : grid_for(n); }
+
 void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int nBinned, int lists,
                          int cap, hipStream_t s) {
   if (n <= 0) return;
@@ -876,8 +883,6 @@ static void density_cap(const Buffers &b, const DevParams &p, const DevScene &s0
 void launch_density(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int cap, hipStream_t s) {
   if (n <= 0) return;
   if (cap == kWideCap) {
-    const int pairs = (int)grid_for(n);
-    hipLaunchKernelGGL(k_reset_minmax, dim3((pairs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b.candStats, pairs);
     if (b.nScenes > 1)
       hipLaunchKernelGGL(k_density_wide<true>, dim3(wide_grid(n)), dim3(kBlock), 0, s, b, p, s0, n);
     else

@@ -91,6 +91,9 @@ void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s
 void launch_scan(const Buffers &b, uint32_t cells, hipStream_t stream);
 void launch_scatter_reorder(const Buffers &b, int n, hipStream_t stream);
 void launch_density(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int cap, hipStream_t stream);
+// Workgroups of the density launch for n particles at capacity cap: the number
+// of (min, max) candidate-count pairs it leaves in candStats.
+uint32_t density_stat_slots(int n, int cap);
 void launch_delta(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int cap, hipStream_t stream);
 // cap value selecting the wide kernels (one wave per particle) of the three
 // pair launchers; results are bit-identical to the other capacities.

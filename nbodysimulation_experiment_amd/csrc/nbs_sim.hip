@@ -318,7 +318,7 @@ class Simulation {
   // Reduces the per-workgroup minima and maxima the last step left behind.
   nbs_stats stats() {
     if (statsPending_ && n_ > 0) {
-      const size_t nc = 2 * (size_t)((n_ + kBlock - 1) / kBlock);
+      const size_t nc = 2 * (size_t)statSlots_;
       const size_t ns = 2 * (size_t)((totalCells_ + kScanChunk - 1) / kScanChunk);
       std::vector<uint32_t> c(nc), s(ns);
       NBS_HIP(hipMemcpyAsync(c.data(), d_.candStats.p, nc * 4, hipMemcpyDeviceToHost, stream_));
@@ -717,7 +717,7 @@ class Simulation {
     grow4(d_.id); grow4(d_.keys); grow4(d_.scn);
     d_.xp.alloc(cap); d_.x1.alloc(cap); d_.id2.alloc(cap); d_.keyp.alloc(cap); d_.fill.alloc(cap);
     d_.tmpSlot.alloc(cap); d_.tmpId.alloc(cap);
-    d_.candStats.alloc(2 * ((cap + kBlock - 1) / kBlock));
+    d_.candStats.alloc(2 * (size_t)density_stat_slots((int)cap, kWideCap));
     cap_ = cap;
     rebindBuffers();
     destroyGraph();
@@ -862,6 +862,7 @@ class Simulation {
     launch_scatter_reorder(buf_, n, stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 2], stream_));
     launch_density(buf_, p, s0, n, cap[1], stream_);
+    statSlots_ = density_stat_slots(n, cap[1]);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 3], stream_));
     if (relax == RELAX_PLAIN) {
       launch_delta(buf_, p, s0, n, cap[2], stream_);
@@ -1107,7 +1108,7 @@ class Simulation {
     S.densA.alloc(c);
     S.densB.alloc(c);
     S.presScratch.alloc(c);
-    S.candScratch.alloc(2 * ((c + kBlock - 1) / kBlock));
+    S.candScratch.alloc(2 * (size_t)density_stat_slots((int)c, kWideCap));
     S.errPartials.alloc(2 * std::max<size_t>((c + 2047) / 2048, 1));
     S.errOut.alloc(2);
     S.pairPart.alloc(surrogate_forward_waves((int)c));
@@ -1335,6 +1336,7 @@ class Simulation {
   nbs_tuning tuning_{1, 64, {2048, 2048, 2048}, 0};
   bool tuned_ = false;
   bool tuneActive_ = false;
+  uint32_t statSlots_ = 0;  // candStats pairs written by the last step's density launch
   uint32_t tuneV_ = 0, tuneDone_ = 0;
   int tuneCand_[3][kNumTileCaps] = {};
   float tuneBest_[3][kNumTileCaps] = {};
