@@ -374,6 +374,52 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
   b.keys[d] = key;
 }
 
+// Exclusive scan of the cell counts in one workgroup, for grids of at most
+// kSmallGridCells cells: one launch instead of k_scan_reduce + k_scan_final,
+// with the same cell starts, count reset and occupied-cell statistics
+// (in cellStats[0..1]).
+__global__ __launch_bounds__(kSmallGridBlock) void k_scan_small(Buffers b, uint32_t cells) {
+  constexpr int kWaves = kSmallGridBlock / 64;
+  __shared__ uint32_t waveTot[kWaves], waveMin[kWaves], waveMax[kWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t run = 0, mn = 0xFFFFFFFFu, mx = 0u;
+  for (uint32_t c0 = 0; c0 < cells; c0 += kSmallGridBlock) {
+    const uint32_t c = c0 + threadIdx.x;
+    const uint32_t v = (c < cells) ? b.count[c] : 0u;
+    mx = max(mx, v);
+    if (v > 0u) mn = min(mn, v);
+    const uint32_t incl = wave_incl_scan(v);
+    if (lane == 63) waveTot[wave] = incl;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const uint32_t t = waveTot[w];
+      if (w < wave) off += t;
+      tot += t;
+    }
+    __syncthreads();
+    if (c < cells) {
+      b.startNext[c] = run + off + incl - v;
+      b.count[c] = 0u;  // ready for the next step's counting
+    }
+    run += tot;
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
+    mn = min(mn, (uint32_t)__shfl_xor((int)mn, d, 64));
+  }
+  if (lane == 0) waveMin[wave] = mn, waveMax[wave] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kWaves; ++w) mn = min(mn, waveMin[w]), mx = max(mx, waveMax[w]);
+    b.cellStats[0] = mn;
+    b.cellStats[1] = mx;
+    b.startNext[cells] = run;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 4. density + pressure
 // ---------------------------------------------------------------------------
@@ -870,6 +916,18 @@ void launch_scatter_reorder(const Buffers &b, int n, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, b, n);
   hipLaunchKernelGGL(k_rank_reorder, dim3(grid_for(n)), dim3(kBlock), 0, s, b, n);
+}
+
+uint32_t launch_grid(const Buffers &b, uint32_t cells, int n, hipStream_t s) {
+  uint32_t slots = 1u;
+  if (cells <= kSmallGridCells) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kSmallGridBlock), 0, s, b, cells);
+  } else {
+    launch_scan(b, cells, s);
+    slots = (cells + kScanChunk - 1) / kScanChunk;
+  }
+  launch_scatter_reorder(b, n, s);
+  return slots;
 }
 
 template <int CAP>

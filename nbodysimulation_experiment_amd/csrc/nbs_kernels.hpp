@@ -9,6 +9,9 @@ namespace nbs {
 constexpr int kBlock = 256;          // threads per workgroup of the particle kernels
 constexpr int kScanItems = 8;        // cells per thread in the scan kernels
 constexpr int kScanChunk = kBlock * kScanItems;
+// One-workgroup cell scan for small grids (k_scan_small).
+constexpr int kSmallGridBlock = 1024;
+constexpr uint32_t kSmallGridCells = 16384;
 
 // Step constants, derived on the host in FP32 exactly as the reference derives
 // them (sph.h:92 invKernelHeight, sph.h:470 h*h, sph.h:492 dt*dt,
@@ -90,6 +93,10 @@ void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s
                          int cap, hipStream_t stream);
 void launch_scan(const Buffers &b, uint32_t cells, hipStream_t stream);
 void launch_scatter_reorder(const Buffers &b, int n, hipStream_t stream);
+// The grid phase: the cell scan (one workgroup for small grids, else
+// launch_scan), then launch_scatter_reorder.  Returns the number of (min, max)
+// occupied-cell pairs it leaves in cellStats.
+uint32_t launch_grid(const Buffers &b, uint32_t cells, int n, hipStream_t stream);
 void launch_density(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int cap, hipStream_t stream);
 // Workgroups of the density launch for n particles at capacity cap: the number
 // of (min, max) candidate-count pairs it leaves in candStats.

@@ -319,7 +319,7 @@ class Simulation {
   nbs_stats stats() {
     if (statsPending_ && n_ > 0) {
       const size_t nc = 2 * (size_t)statSlots_;
-      const size_t ns = 2 * (size_t)((totalCells_ + kScanChunk - 1) / kScanChunk);
+      const size_t ns = 2 * (size_t)cellSlots_;
       std::vector<uint32_t> c(nc), s(ns);
       NBS_HIP(hipMemcpyAsync(c.data(), d_.candStats.p, nc * 4, hipMemcpyDeviceToHost, stream_));
       NBS_HIP(hipMemcpyAsync(s.data(), d_.cellStats.p, ns * 4, hipMemcpyDeviceToHost, stream_));
@@ -858,8 +858,7 @@ class Simulation {
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 0], stream_));
     launch_visc_predict(buf_, p, s0, n, (int)nBinned_, lists_ ? 1 : 0, cap[0], stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 1], stream_));
-    launch_scan(buf_, totalCells_, stream_);
-    launch_scatter_reorder(buf_, n, stream_);
+    cellSlots_ = launch_grid(buf_, totalCells_, n, stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 2], stream_));
     launch_density(buf_, p, s0, n, cap[1], stream_);
     statSlots_ = density_stat_slots(n, cap[1]);
@@ -1337,6 +1336,7 @@ class Simulation {
   bool tuned_ = false;
   bool tuneActive_ = false;
   uint32_t statSlots_ = 0;  // candStats pairs written by the last step's density launch
+  uint32_t cellSlots_ = 0;  // cellStats pairs written by the last step's grid phase
   uint32_t tuneV_ = 0, tuneDone_ = 0;
   int tuneCand_[3][kNumTileCaps] = {};
   float tuneBest_[3][kNumTileCaps] = {};
