@@ -64,43 +64,89 @@ __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const D
 
 // Wide pair walk for small particle counts: one wave per particle.  Lane t
 // tests candidate base + t of a 64-candidate chunk and, when the pair is in
-// range, term() computes the pair's K terms and says whether it contributes.
-// The contributions are then applied by add() in candidate order, one lane's
-// terms at a time (broadcast with readlane), so the accumulation performs the
-// same FP32 operations in the same order as for_each_pair; only the per-pair
-// arithmetic runs in parallel across the wave.  Returns the candidate count.
-template <int K, typename Idx, typename Term, typename Add>
+// range, term(j, Rx, Ry, r2, G[j], t) computes the pair's K terms and says
+// whether it contributes.  The contributions are then applied in candidate
+// order to two accumulators held in lanes 0 and 1: for each contributing pair
+// and each of STAGES stages s, lane c in {0, 1} does acc -= t[(2s + c) % K]
+// (SUB) or acc += t[(2s + c) % K], reading the terms back from the wave's LDS
+// rows (lds: 65 * K floats; row 64 holds the neutral value +0 for SUB, -0
+// otherwise, which leaves an accumulator unchanged bit for bit and pads groups
+// of four pairs).  So the accumulation performs the same FP32 operations in
+// the same order as the one-lane-per-particle walk; only the per-pair
+// arithmetic runs in parallel across the wave.  acc0 and acc1 are the
+// accumulators' initial and final values (the same in every lane).  Each
+// chunk's position (and G value) loads are issued one chunk ahead.  Returns
+// the candidate count.
+template <int K, int STAGES, bool SUB, typename Idx, typename Term>
 __device__ __forceinline__ uint32_t for_each_pair_wide(const uint32_t *start, const DevScene &sc, int cx, int cy,
-                                                       const float2 *X, Idx idx, float2 xi, float h2, Term &&term,
-                                                       Add &&add) {
-  const int lane = threadIdx.x & 63;
-  uint32_t cand = 0;
-  for (int d = 0; d < 3; ++d) {
-    const RowRange rr = row_range(start, sc, cx, cy, d - 1);
-    cand += rr.j1 - rr.j0;
-    for (uint32_t base = rr.j0; base < rr.j1; base += 64u) {
-      const uint32_t j = base + (uint32_t)lane;
-      float t[K];
+                                                       const float2 *X, Idx idx, float2 xi, float h2, const float2 *G,
+                                                       float *lds, float &acc0, float &acc1, Term &&term) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t c = lane & 1u;
+  if (lane < (uint32_t)K) lds[64 * K + lane] = SUB ? 0.0f : -0.0f;
+  float acc = c ? acc1 : acc0;
+  const RowRange ra = row_range(start, sc, cx, cy, -1), rb = row_range(start, sc, cx, cy, 0),
+                 rc = row_range(start, sc, cx, cy, 1);
+  const uint32_t cand = (ra.j1 - ra.j0) + (rb.j1 - rb.j0) + (rc.j1 - rc.j0);
+  int d = 0;
+  uint32_t base = ra.j0, end = ra.j1;
+  auto skip_done_rows = [&]() {
+    while (d < 3 && base >= end) {
+      ++d;
+      if (d == 1) base = rb.j0, end = rb.j1;
+      if (d == 2) base = rc.j0, end = rc.j1;
+    }
+  };
+  float2 xn = make_float2(0.0f, 0.0f), gn = make_float2(0.0f, 0.0f);
+  bool inN = false;
+  auto issue = [&]() {
+    inN = d < 3 && base + lane < end;
+    if (inN) {
+      xn = X[idx(d, base) + lane];
+      if (G) gn = G[base + lane];
+    }
+  };
+  skip_done_rows();
+  issue();
+  while (d < 3) {
+    const float2 xj = xn, gj = gn;
+    const bool in = inN;
+    const uint32_t j = base + lane;
+    base += 64u;
+    skip_done_rows();
+    issue();
+    float t[K];
 #pragma unroll
-      for (int k = 0; k < K; ++k) t[k] = 0.0f;
-      bool ok = false;
-      if (j < rr.j1) {
-        const float2 xj = X[idx(d, base) + (uint32_t)lane];
-        const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;
-        const float r2 = Rx * Rx + Ry * Ry;
-        if (r2 < h2) ok = term(j, Rx, Ry, r2, t);
-      }
-      uint64_t mask = __ballot(ok);
-      while (mask) {
-        const int src = __builtin_ctzll(mask);
-        mask &= mask - 1u;
-        float v[K];
+    for (int k = 0; k < K; ++k) t[k] = 0.0f;
+    bool ok = false;
+    if (in) {
+      const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;
+      const float r2 = Rx * Rx + Ry * Ry;
+      if (r2 < h2) ok = term(j, Rx, Ry, r2, gj, t);
+    }
 #pragma unroll
-        for (int k = 0; k < K; ++k) v[k] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t[k]), src));
-        add(v);
+    for (int k = 0; k < K; ++k) lds[lane * K + k] = t[k];
+    uint64_t mask = __ballot(ok);
+    while (mask) {
+      uint32_t q[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        q[u] = mask ? (uint32_t)__builtin_ctzll(mask) : 64u;
+        mask &= mask - 1u;  // no-op once empty
       }
+      float v[4][STAGES];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int st = 0; st < STAGES; ++st) v[u][st] = lds[q[u] * K + (2 * st + c) % K];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int st = 0; st < STAGES; ++st) acc = SUB ? acc - v[u][st] : acc + v[u][st];
     }
   }
+  acc0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(acc), 0));
+  acc1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(acc), 1));
   return cand;
 }
 
@@ -584,12 +630,12 @@ __global__ __launch_bounds__(kBlock) void k_visc_predict_wide(Buffers b, DevPara
     const uint32_t local = b.keys[i] - sc.cellBase;
     const int cy = (int)(local / (uint32_t)sc.nx);
     const int cx = (int)local - cy * sc.nx;
+    __shared__ float lds[kWidePerBlock][65 * 2];
     float accx = 0.0f, accy = 0.0f;
-    const float2 *V = b.v;
-    for_each_pair_wide<2>(
-        b.startCur, sc, cx, cy, b.x0, SlotIndex(), xi, p.h2,
-        [&](uint32_t j, float Rx, float Ry, float r2, float *t) {
-          const float2 vj = V[j];
+    // per pair: accx -= ix, accy -= iy (own visit), then again (neighbour's visit)
+    for_each_pair_wide<2, 2, true>(
+        b.startCur, sc, cx, cy, b.x0, SlotIndex(), xi, p.h2, b.v, lds[threadIdx.x >> 6], accx, accy,
+        [&](uint32_t, float Rx, float Ry, float r2, float2 vj, float *t) {
           const float vjx = p.gdtx + vj.x, vjy = p.gdty + vj.y;
           float r, inv;
           pair_length(r2, r, inv);                              // sph.h:503, 505
@@ -601,12 +647,6 @@ __global__ __launch_bounds__(kBlock) void k_visc_predict_wide(Buffers b, DevPara
           t[0] = ((f * nX) * 0.5f) * p.dt;                      // demo4.cpp:233
           t[1] = ((f * nY) * 0.5f) * p.dt;
           return true;
-        },
-        [&](const float *v) {
-          accx = accx - v[0];                                   // own visit
-          accy = accy - v[1];
-          accx = accx - v[0];                                   // neighbour's visit
-          accy = accy - v[1];
         });
     vx = accx + vx;
     vy = accy + vy;
@@ -623,13 +663,12 @@ __global__ __launch_bounds__(kBlock) void k_visc_predict_wide(Buffers b, DevPara
   }
 }
 
-// (min, max) candidate-count pairs of the one-lane-per-particle workgroups,
-// reset before the wide density kernel folds its counts in.
 // Candidate-count statistics: one min/max pair per workgroup (kWidePerBlock
 // particles), slot blockIdx.x.
 template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_density_wide(Buffers b, DevParams p, DevScene s0, int n) {
   __shared__ uint32_t shCand[kWidePerBlock];
+  __shared__ float lds[kWidePerBlock][65 * 2];
   const int w = (int)(threadIdx.x >> 6);
   const int i = (int)(blockIdx.x * kWidePerBlock + w);
   if (i < n) {  // whole wave
@@ -640,18 +679,15 @@ __global__ __launch_bounds__(kBlock) void k_density_wide(Buffers b, DevParams p,
     const int cx = (int)local - cy * sc.nx;
     const float2 xi = b.x1[i];
     float d0 = 0.0f, d1 = 0.0f;
-    const uint32_t cand = for_each_pair_wide<2>(
-        b.startNext, sc, cx, cy, b.x1, SlotIndex(), xi, p.h2,
-        [&](uint32_t, float, float, float r2, float *t) {
+    // per pair: d0 += term^2, d1 += term^3 (sph.h:473-474)
+    const uint32_t cand = for_each_pair_wide<2, 1, false>(
+        b.startNext, sc, cx, cy, b.x1, SlotIndex(), xi, p.h2, nullptr, lds[w], d0, d1,
+        [&](uint32_t, float, float, float r2, float2, float *t) {
           const float r = pair_sqrt(r2);                        // sph.h:471
           const float term = 1.0f - r * p.invH;                 // sph.h:472
           t[0] = term * term;
           t[1] = (term * term) * term;
           return true;
-        },
-        [&](const float *v) {
-          d0 = d0 + v[0];                                       // sph.h:473
-          d1 = d1 + v[1];                                       // sph.h:474
         });
     if ((threadIdx.x & 63) == 0) {
       b.dens[i] = make_float2(d0, d1);
@@ -682,12 +718,13 @@ __global__ __launch_bounds__(kBlock) void k_delta_wide(Buffers b, DevParams p, D
   const int cx = (int)local - cy * sc.nx;
   const float2 xi = b.x1[i];
   const float2 pi = b.pres[i];
-  const float2 *P = b.pres;
+  __shared__ float lds[kWidePerBlock][65 * 4];
   float accx = 0.0f, accy = 0.0f;
-  for_each_pair_wide<4>(
-      b.startNext, sc, cx, cy, b.x1, SlotIndex(), xi, p.h2,
-      [&](uint32_t j, float Rx, float Ry, float r2, float *t) {
-        const float2 pj = P[j];
+  // per pair: accx -= t0, accy -= t1 (demo4.cpp:251), then accx -= t2, accy -= t3
+  // (demo4.cpp:250, j's visit)
+  for_each_pair_wide<4, 2, true>(
+      b.startNext, sc, cx, cy, b.x1, SlotIndex(), xi, p.h2, b.pres, lds[threadIdx.x >> 6], accx, accy,
+      [&](uint32_t, float Rx, float Ry, float r2, float2 pj, float *t) {
         float r, inv;
         pair_length(r2, r, inv);                                // sph.h:489-490
         const float nX = Rx * inv, nY = Ry * inv;
@@ -700,12 +737,6 @@ __global__ __launch_bounds__(kBlock) void k_delta_wide(Buffers b, DevParams p, D
         t[2] = (dji * nX) * 0.5f;
         t[3] = (dji * nY) * 0.5f;
         return true;
-      },
-      [&](const float *v) {
-        accx = accx - v[0];                                     // demo4.cpp:251
-        accy = accy - v[1];
-        accx = accx - v[2];                                     // demo4.cpp:250 (j's visit)
-        accy = accy - v[3];
       });
   // Every lane holds the same sums: the wave walks the bodies once for lane 0.
   float x = accx + xi.x;                                        // demo4.cpp:253
