@@ -658,9 +658,17 @@ __global__ __launch_bounds__(kBlock) void k_visc_predict_wide(Buffers b, DevPara
     const int cx = cell_coord(xpx, sc.halfW, sc.cellSize, sc.nx);
     const int cy = cell_coord(xpy, sc.halfH, sc.cellSize, sc.ny);
     const uint32_t key = sc.cellBase + (uint32_t)(cy * sc.nx + cx);
-    b.keyp[i] = key;
-    b.fill[i] = atomicAdd(&b.count[key], 1u);  // arrival slot; the rank pass orders the cell
+    b.keyp[i] = key;  // counted by k_count_fill
   }
+}
+
+// Cell counting after the wide viscosity kernel: one lane per particle, so
+// particles of one cell share wave-aggregated atomics instead of one atomic
+// per particle on the same counter.
+__global__ __launch_bounds__(kBlock) void k_count_fill(Buffers b, int n) {
+  const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (i >= n) return;
+  b.fill[i] = wave_aggregated_fill(b.count, b.keyp[i]);  // arrival slot; the rank pass orders the cell
 }
 
 // Candidate-count statistics: one min/max pair per workgroup (kWidePerBlock
@@ -927,6 +935,7 @@ void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s
     else
       hipLaunchKernelGGL(k_visc_predict_wide<false>, dim3(wide_grid(n)), dim3(kBlock), 0, s, b, p, s0, n, nBinned,
                          lists);
+    hipLaunchKernelGGL(k_count_fill, dim3(grid_for(n)), dim3(kBlock), 0, s, b, n);
     return;
   }
   switch (cap) {
