@@ -411,8 +411,12 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
   const uint32_t myId = b.tmpId[p];
   const uint32_t key = b.keyp[s];
   const uint32_t c0 = b.startNext[key], c1 = b.startNext[key + 1];
-  uint32_t rank = 0;
-  for (uint32_t q = c0; q < c1; ++q) rank += (b.tmpId[q] < myId) ? 1u : 0u;
+  uint32_t rank = 0, q = c0;
+  for (; q + 4u <= c1; q += 4u) {  // four independent loads in flight
+    const uint32_t a0 = b.tmpId[q], a1 = b.tmpId[q + 1], a2 = b.tmpId[q + 2], a3 = b.tmpId[q + 3];
+    rank += (a0 < myId ? 1u : 0u) + (a1 < myId ? 1u : 0u) + (a2 < myId ? 1u : 0u) + (a3 < myId ? 1u : 0u);
+  }
+  for (; q < c1; ++q) rank += (b.tmpId[q] < myId) ? 1u : 0u;
   const uint32_t d = c0 + rank;
   b.x1[d] = b.xp[s];
   b.prev[d] = b.x0[s];  // demo4.cpp:334 prevPosition = curPosition
@@ -426,31 +430,21 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
 // (in cellStats[0..1]).
 __global__ __launch_bounds__(kSmallGridBlock) void k_scan_small(Buffers b, uint32_t cells) {
   constexpr int kWaves = kSmallGridBlock / 64;
+  constexpr uint32_t kItems = kSmallGridCells / kSmallGridBlock;  // consecutive cells per thread
   __shared__ uint32_t waveTot[kWaves], waveMin[kWaves], waveMax[kWaves];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint32_t run = 0, mn = 0xFFFFFFFFu, mx = 0u;
-  for (uint32_t c0 = 0; c0 < cells; c0 += kSmallGridBlock) {
-    const uint32_t c = c0 + threadIdx.x;
-    const uint32_t v = (c < cells) ? b.count[c] : 0u;
-    mx = max(mx, v);
-    if (v > 0u) mn = min(mn, v);
-    const uint32_t incl = wave_incl_scan(v);
-    if (lane == 63) waveTot[wave] = incl;
-    __syncthreads();
-    uint32_t off = 0, tot = 0;
+  const uint32_t c0 = threadIdx.x * kItems;
+  uint32_t v[kItems];
+  uint32_t sum = 0, mn = 0xFFFFFFFFu, mx = 0u;
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-      const uint32_t t = waveTot[w];
-      if (w < wave) off += t;
-      tot += t;
-    }
-    __syncthreads();
-    if (c < cells) {
-      b.startNext[c] = run + off + incl - v;
-      b.count[c] = 0u;  // ready for the next step's counting
-    }
-    run += tot;
+  for (uint32_t k = 0; k < kItems; ++k) {
+    v[k] = (c0 + k < cells) ? b.count[c0 + k] : 0u;
+    sum += v[k];
+    mx = max(mx, v[k]);
+    if (v[k] > 0u) mn = min(mn, v[k]);
   }
+  const uint32_t incl = wave_incl_scan(sum);
+  if (lane == 63) waveTot[wave] = incl;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) {
     mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
@@ -458,11 +452,27 @@ __global__ __launch_bounds__(kSmallGridBlock) void k_scan_small(Buffers b, uint3
   }
   if (lane == 0) waveMin[wave] = mn, waveMax[wave] = mx;
   __syncthreads();
+  uint32_t run = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    const uint32_t t = waveTot[w];
+    if (w < wave) run += t;
+    tot += t;
+  }
+  run += incl - sum;
+#pragma unroll
+  for (uint32_t k = 0; k < kItems; ++k) {
+    if (c0 + k < cells) {
+      b.startNext[c0 + k] = run;
+      b.count[c0 + k] = 0u;  // ready for the next step's counting
+    }
+    run += v[k];
+  }
   if (threadIdx.x == 0) {
     for (int w = 1; w < kWaves; ++w) mn = min(mn, waveMin[w]), mx = max(mx, waveMax[w]);
     b.cellStats[0] = mn;
     b.cellStats[1] = mx;
-    b.startNext[cells] = run;
+    b.startNext[cells] = tot;
   }
 }
 
