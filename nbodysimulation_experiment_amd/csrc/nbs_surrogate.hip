@@ -706,34 +706,61 @@ __global__ __launch_bounds__(kBlock) void k_dens_err_partial(const float2 *a, co
 }
 
 // Per-scene sums of |a.x - b.x| and b.x into out[2 s], out[2 s + 1] (zeroed
-// beforehand).  A scene's particles occupy consecutive slots, so a wave
-// whose lanes share one scene reduces first and adds once.
+// beforehand).  A scene's particles occupy consecutive slots.  A workgroup
+// covers kErrItems * kBlock consecutive slots; when they all belong to one
+// scene (the first and last slot share it) it reduces and adds once, else each
+// wave adds per 64-slot group (once when the group shares a scene, per lane
+// otherwise).
 __global__ __launch_bounds__(kBlock) void k_dens_err_scenes(const float2 *a, const float2 *bb, const uint32_t *keys,
                                                             const uint32_t *sceneBase, int nScenes, int n,
                                                             double *out) {
-  const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
-  const bool valid = i < n;
+  __shared__ double red[2][kBlock / 64];
+  const int base = (int)blockIdx.x * kBlock * kErrItems;
+  const int last = min(n, base + kBlock * kErrItems) - 1;
+  const int s0 = scene_of_key(sceneBase, nScenes, keys[base]);
+  const bool one = scene_of_key(sceneBase, nScenes, keys[last]) == s0;
   double e = 0.0, r = 0.0;
-  int sc = -1;
-  if (valid) {
-    e = fabs((double)a[i].x - (double)bb[i].x);
-    r = (double)bb[i].x;
-    sc = scene_of_key(sceneBase, nScenes, keys[i]);
+  for (int k = 0; k < kErrItems; ++k) {
+    const int i = base + k * kBlock + (int)threadIdx.x;
+    const bool valid = i < n;
+    double ei = 0.0, ri = 0.0;
+    if (valid) {
+      ei = fabs((double)a[i].x - (double)bb[i].x);
+      ri = (double)bb[i].x;
+    }
+    if (one) {
+      e += ei;
+      r += ri;
+      continue;
+    }
+    const int sc = valid ? scene_of_key(sceneBase, nScenes, keys[i]) : -1;
+    const int first = __shfl(sc, 0, 64);
+    if (__ballot(valid && sc != first) == 0ull) {
+      for (int off = 32; off > 0; off >>= 1) {
+        ei += __shfl_xor(ei, off, 64);
+        ri += __shfl_xor(ri, off, 64);
+      }
+      if ((threadIdx.x & 63) == 0 && first >= 0) {
+        atomicAdd(&out[2 * first], ei);
+        atomicAdd(&out[2 * first + 1], ri);
+      }
+    } else if (valid) {
+      atomicAdd(&out[2 * sc], ei);
+      atomicAdd(&out[2 * sc + 1], ri);
+    }
   }
-  const int first = __shfl(sc, 0, 64);
-  const bool uniform = __ballot(valid && sc != first) == 0ull;
-  if (uniform) {
-    for (int off = 32; off > 0; off >>= 1) {
-      e += __shfl_xor(e, off, 64);
-      r += __shfl_xor(r, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0 && first >= 0) {
-      atomicAdd(&out[2 * first], e);
-      atomicAdd(&out[2 * first + 1], r);
-    }
-  } else if (valid) {
-    atomicAdd(&out[2 * sc], e);
-    atomicAdd(&out[2 * sc + 1], r);
+  if (!one) return;  // uniform across the workgroup
+  for (int off = 32; off > 0; off >>= 1) {
+    e += __shfl_xor(e, off, 64);
+    r += __shfl_xor(r, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) red[0][threadIdx.x >> 6] = e, red[1][threadIdx.x >> 6] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t0 = 0.0, t1 = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) t0 += red[0][w], t1 += red[1][w];
+    atomicAdd(&out[2 * s0], t0);
+    atomicAdd(&out[2 * s0 + 1], t1);
   }
 }
 
@@ -742,12 +769,24 @@ __global__ __launch_bounds__(kBlock) void k_fill_f64(double *p, int count) {
   if (i < count) p[i] = 0.0;
 }
 
-__global__ void k_dens_err_final(const double *partials, int nb, double *out) {
-  if (threadIdx.x != 0) return;
+// Sums the partials in a fixed order: thread t adds partials t, t + 1024, ...,
+// then a tree over the threads.
+__global__ __launch_bounds__(1024) void k_dens_err_final(const double *partials, int nb, double *out) {
+  __shared__ double red[2][1024 / 64];
   double t0 = 0.0, t1 = 0.0;
-  for (int k = 0; k < nb; ++k) t0 += partials[2 * k], t1 += partials[2 * k + 1];
-  out[0] = t0;
-  out[1] = t1;
+  for (int k = (int)threadIdx.x; k < nb; k += 1024) t0 += partials[2 * k], t1 += partials[2 * k + 1];
+  for (int off = 32; off > 0; off >>= 1) {
+    t0 += __shfl_xor(t0, off, 64);
+    t1 += __shfl_xor(t1, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) red[0][threadIdx.x >> 6] = t0, red[1][threadIdx.x >> 6] = t1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int w = 0; w < 1024 / 64; ++w) s0 += red[0][w], s1 += red[1][w];
+    out[0] = s0;
+    out[1] = s1;
+  }
 }
 
 // MFMA layout self-test (see the file header).
@@ -843,15 +882,15 @@ void launch_adam(float *params, const float *grad, float *mo, float *ve, int cou
 void launch_density_error(const float2 *a, const float2 *bb, int n, double *partials, double *out, hipStream_t s) {
   const int nb = std::max(1, (n + kBlock * kErrItems - 1) / (kBlock * kErrItems));
   if (n > 0) hipLaunchKernelGGL(k_dens_err_partial, dim3(nb), dim3(kBlock), 0, s, a, bb, n, partials);
-  hipLaunchKernelGGL(k_dens_err_final, dim3(1), dim3(64), 0, s, (const double *)partials, n > 0 ? nb : 0, out);
+  hipLaunchKernelGGL(k_dens_err_final, dim3(1), dim3(1024), 0, s, (const double *)partials, n > 0 ? nb : 0, out);
 }
 
 void launch_density_error_scenes(const float2 *a, const float2 *bb, const uint32_t *keys, const uint32_t *sceneBase,
                                  int nScenes, int n, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_f64, dim3((2 * nScenes + kBlock - 1) / kBlock), dim3(kBlock), 0, s, out, 2 * nScenes);
   if (n > 0)
-    hipLaunchKernelGGL(k_dens_err_scenes, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, bb, keys, sceneBase,
-                       nScenes, n, out);
+    hipLaunchKernelGGL(k_dens_err_scenes, dim3((n + kBlock * kErrItems - 1) / (kBlock * kErrItems)), dim3(kBlock), 0,
+                       s, a, bb, keys, sceneBase, nScenes, n, out);
 }
 
 void launch_mfma_probe(const float *a, const float *b, float *c, hipStream_t s) {
