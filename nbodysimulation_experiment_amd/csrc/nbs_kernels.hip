@@ -42,14 +42,29 @@ __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const D
     for (uint32_t base = rr.j0; base < rr.j1; base += 32u) {
       const uint32_t cnt = min(32u, rr.j1 - base);
       const uint32_t q0 = idx(d, base);
+      // mask = 2 * mask + (r2 < h2), last candidate first, so candidate t
+      // ends at bit t: the compare's lane mask is the add's carry-in.
       uint32_t mask = 0;
-#pragma unroll 4
-      for (uint32_t t = 0; t < cnt; ++t) {
+      auto push = [&](float r2) {
+        asm("v_cmp_gt_f32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
+            : "+v"(mask)
+            : "s"(h2), "v"(r2)
+            : "vcc");
+      };
+      auto dist2 = [&](uint32_t t) {
         const float2 xj = X[q0 + t];
         const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;
-        const float r2 = Rx * Rx + Ry * Ry;
-        mask |= (r2 < h2) ? (1u << t) : 0u;
+        return Rx * Rx + Ry * Ry;
+      };
+      uint32_t u = cnt;
+      for (; u >= 4u; u -= 4u) {
+        const float a = dist2(u - 1), b = dist2(u - 2), c = dist2(u - 3), e = dist2(u - 4);
+        push(a);
+        push(b);
+        push(c);
+        push(e);
       }
+      for (; u > 0u; --u) push(dist2(u - 1));
       while (mask) {
         const uint32_t t = (uint32_t)__builtin_ctz(mask);
         mask &= mask - 1u;
