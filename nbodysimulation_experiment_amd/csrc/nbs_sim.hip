@@ -842,7 +842,13 @@ class Simulation {
 
   // Pair-kernel variant per phase (viscosity, density, relaxation): an LDS
   // tile capacity, or kWideCap for small steps.
-  bool wideStep() const { return tuning_.wide_max_particles > 0 && n_ <= (uint64_t)tuning_.wide_max_particles; }
+  // Wide kernels for every phase: below the autotuner's range (a few hundred
+  // particles, where one lane per particle fills a single workgroup) or up to
+  // nbs_tuning.wide_max_particles.
+  bool wideStep() const {
+    return (tuning_.autotune && n_ < kTuneMinParticles) ||
+           (tuning_.wide_max_particles > 0 && n_ <= (uint64_t)tuning_.wide_max_particles);
+  }
   void pairCaps(int cap[3]) const {
     for (int k = 0; k < 3; ++k) cap[k] = wideStep() ? kWideCap : tuning_.tile_cap[k];
   }
