@@ -60,6 +60,10 @@ def build_workload(name):
     if name == "batched-256":
         sp = scenes.batched_scenes()
         return sp, f"{len(sp)} independent scenes, {sum(s.n for s in sp):,} particles, counts log-uniform in [2048, 65536]"
+    if name == "batched-surrogate":
+        sp = scenes.batched_scenes() + [scenes.dam_break_4m()]
+        return sp, (f"{len(sp) - 1} independent scenes (counts log-uniform in [2048, 65536]) plus one 4,194,304-particle "
+                    f"dam break (2048x2048 at 0.5h, 1536h x 1536h box), {sum(s.n for s in sp):,} particles")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -172,7 +176,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="dam-break-1m",
-                    choices=["dam-break-4k", "dam-break-1m", "batched-256", "scenarios"])
+                    choices=["dam-break-4k", "dam-break-1m", "batched-256", "batched-surrogate", "scenarios"])
     ap.add_argument("--scenario-cpu-iterations", type=int, default=2)
     ap.add_argument("--graph", action="store_true", help="replay steps as HIP graphs (no per-kernel timing)")
     ap.add_argument("--cpu-steps", type=int, default=40)
@@ -278,7 +282,8 @@ def main():
                     "step_alg_bytes_frac": round(value * STEP_ALG_BYTES / 1e9 / HBM_PEAK_GBS, 5)}
 
     surrogate = None
-    if args.workload == "dam-break-1m" and not args.no_surrogate and not args.graph:
+    online = args.workload == "batched-surrogate"  # in-run training on each check step's plain frame
+    if args.workload in ("dam-break-1m", "batched-surrogate") and not args.no_surrogate and not args.graph:
         log(f"training the learned relaxation: {args.train_scenes} scenes, {args.train_steps} steps")
         params, train = train_surrogate(args)
         log(f"training: {train}")
@@ -290,6 +295,8 @@ def main():
         ssim = make_sim()
         ssim.surrogate_set_params(params)
         ssim.surrogate_set_mode(True, args.check_interval, args.max_density_error)
+        if online:
+            ssim.surrogate_set_online_training(args.lr)
         ssim.set_timing(True)
         ssim.ResetStats()
         log(f"timing {args.steps} surrogate steps")
@@ -299,6 +306,7 @@ def main():
         s_kt = ssim.kernel_times()
         rep = ssim.surrogate_report()
         sdiag = ssim.diagnostics()
+        scene_err = ssim.surrogate_scene_errors() if online else None
         ssim.close()
         s_value = n * args.steps / s_elapsed
         relax_ms = s_kt["delta"][0]
@@ -320,6 +328,12 @@ def main():
                      "note": "pair layer 2*64*16 flops per pair + head 2*2*64*64 per particle, over the relaxation "
                              "phase's average time (check steps add a plain relaxation and two density passes)"},
             "training": train,
+            "in_run_training": None if not online else {
+                "learning_rate": args.lr, "adam_steps": rep["train_steps"],
+                "scene_density_error": {"scenes": int(scene_err.size), "min": float(scene_err.min()),
+                                        "max": float(scene_err.max()), "mean": float(scene_err.mean()),
+                                        "largest_scene": float(scene_err[-1]),
+                                        "note": "last check step, per scene; the last scene is the 4M dam break"}},
             "kernels_ms": {k: round(v[0], 5) for k, v in s_kt.items()},
             "diagnostics_fp64": {k: sdiag[k] for k in ("mean_density", "max_density", "non_finite_count")},
         }

@@ -187,6 +187,16 @@ def dam_break_1m(seed: int = 1, n_side: int = 1024, n_circles: int = 64) -> Scen
     return SceneSpec(width, height, 1.0, pos, planes=wall_planes(width, height), circles=circles)
 
 
+def dam_break_4m(seed: int = 4, n_side: int = 2048) -> SceneSpec:
+    """4,194,304 particles (2048x2048 at 0.5h) against the left wall and floor of
+    a 1536h x 1536h box with 4 walls."""
+    width = height = 1536.0
+    side = n_side * 0.5
+    center = (-width * 0.5 + side * 0.5, -height * 0.5 + side * 0.5)
+    pos = volume_positions(center, n_side, n_side, 0.5, 1.0, seed)
+    return SceneSpec(width, height, 1.0, pos, planes=wall_planes(width, height))
+
+
 def batched_scenes(seed: int = 2, count: int = 256, n_min: int = 2048, n_max: int = 65536,
                    box_lo: float = 1.5, box_hi: float = 3.0) -> list[SceneSpec]:
     """Independent scenes with particle counts log-uniform in [n_min, n_max];
