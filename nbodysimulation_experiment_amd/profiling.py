@@ -22,19 +22,24 @@ import re
 import sys
 
 SHORT = [
-    ("k_visc_predict", "viscosity_predict"), ("k_scan_reduce", "scan_reduce"), ("k_scan_blocks", "scan_blocks"),
+    ("k_visc_predict_wide", "viscosity_predict_wide"), ("k_density_wide", "density_wide"),
+    ("k_delta_wide", "delta_wide"),
+    ("k_visc_predict", "viscosity_predict"), ("k_count_fill", "count_fill"), ("k_scan_reduce", "scan_reduce"),
+    ("k_scan_small", "scan_small"),
     ("k_scan_final", "scan_final"), ("k_scatter_grad", "scatter_grad"), ("k_scatter", "scatter"), ("k_rank_reorder", "rank_reorder"),
     ("k_density", "density"), ("k_delta", "delta"), ("k_copy", "copy"), ("k_sur_forward", "sur_forward"),
     ("k_sur_back_pairs", "sur_back_pairs"), ("k_sur_back_head", "sur_back_head"),
 ]
 
-# Kernel groups of the step (timed together by the library's events).
+# Kernel groups of the step (timed together by the library's events).  Each
+# group lists its alternatives: the kernels one step launches for the phase in
+# the one-lane-per-particle and in the wide form.
 GROUPS = {
-    "viscosity_predict": ("viscosity_predict",),
-    "grid": ("scan_reduce", "scan_final", "scatter", "rank_reorder"),
-    "density": ("density",),
-    "delta": ("delta",),
-    "surrogate_forward": ("sur_forward",),
+    "viscosity_predict": (("viscosity_predict",), ("viscosity_predict_wide", "count_fill")),
+    "grid": (("scan_reduce", "scan_final", "scatter", "rank_reorder"), ("scan_small", "scatter", "rank_reorder")),
+    "density": (("density",), ("density_wide",)),
+    "delta": (("delta",), ("delta_wide",)),
+    "surrogate_forward": (("sur_forward",),),
 }
 
 
@@ -85,19 +90,21 @@ def traffic(fetch_csv: str, write_csv: str, copy_bytes: float, workload: str) ->
     ff = copy_bytes / (sum(f["copy"]) / len(f["copy"]) * 1024.0)
     wf = copy_bytes / (sum(w["copy"]) / len(w["copy"]) * 1024.0)
     kernels, detail = {}, {}
-    for g, members in GROUPS.items():
-        tot_f = tot_w = 0.0
-        present = False
-        for k in members:
-            if k in f and k in w:
-                present = True
-                mf = sum(f[k]) / len(f[k]) * 1024.0 * ff
-                mw = sum(w[k]) / len(w[k]) * 1024.0 * wf
-                detail[k] = {"fetch_bytes": round(mf), "write_bytes": round(mw), "dispatches": len(f[k])}
-                tot_f += mf
-                tot_w += mw
-        if present:
-            kernels[g] = round(tot_f + tot_w)
+    for k in f:
+        if k in w and k != "copy":
+            detail[k] = {"fetch_bytes": round(sum(f[k]) / len(f[k]) * 1024.0 * ff),
+                         "write_bytes": round(sum(w[k]) / len(w[k]) * 1024.0 * wf), "dispatches": len(f[k])}
+    for g, variants in GROUPS.items():
+        # bytes per step of each alternative that ran, weighted by its launches
+        tot = weight = 0.0
+        for members in variants:
+            if not all(k in detail for k in members):
+                continue
+            launches = detail[members[0]]["dispatches"]
+            tot += launches * sum(detail[k]["fetch_bytes"] + detail[k]["write_bytes"] for k in members)
+            weight += launches
+        if weight:
+            kernels[g] = round(tot / weight)
     return {"workload": workload, "unit": "bytes per launch", "fetch_factor": round(ff, 4),
             "write_factor": round(wf, 4), "kernels": kernels, "detail": detail}
 
