@@ -110,6 +110,7 @@ struct Cursor {
   uint32_t j, jend, tb;
   int d;
   bool done;
+  uint32_t mask, mbase;  // in-range candidates of the current chunk: bit b is slot mbase + b
 };
 
 // Neighbour data of a wave's candidates: an LDS copy of the three candidate
@@ -176,27 +177,41 @@ __device__ __forceinline__ PairSrc stage_pairs(const Buffers &b, const DevScene 
 }
 
 // Advances to the lane's next pair (j != i, r^2 < h^2); false when exhausted.
-// t is the pair's index in src.
+// t is the pair's index in src.  Candidates are tested kWalkChunk at a time
+// into a mask, whose bits are then taken lowest first (candidate order).
+constexpr uint32_t kWalkChunk = 8;
 __device__ __forceinline__ bool next_pair(Cursor &cu, const Buffers &b, const DevParams &p, const PairSrc &src,
                                           const PartInfo &q, int i, uint32_t &t, float2 &xj, float &Rx, float &Ry,
                                           float &r2) {
-  while (!cu.done) {
-    if (cu.j >= cu.jend && !cursor_next_row(cu, b.startNext, q, src)) {
-      cu.done = true;
-      break;
-    }
-    const uint32_t j = cu.j++;
-    const uint32_t k = j + cu.tb;
-    xj = src.x[k];
-    Rx = xj.x - q.xi.x;
-    Ry = xj.y - q.xi.y;
-    r2 = Rx * Rx + Ry * Ry;
-    if (r2 < p.h2 && j != (uint32_t)i) {
-      t = k;
+  for (;;) {
+    if (cu.mask) {
+      const uint32_t bit = (uint32_t)__builtin_ctz(cu.mask);
+      cu.mask &= cu.mask - 1u;
+      t = cu.mbase + bit + cu.tb;
+      xj = src.x[t];
+      Rx = xj.x - q.xi.x;
+      Ry = xj.y - q.xi.y;
+      r2 = Rx * Rx + Ry * Ry;
       return true;
     }
+    if (cu.done) return false;
+    if (cu.j >= cu.jend && !cursor_next_row(cu, b.startNext, q, src)) {
+      cu.done = true;
+      return false;
+    }
+    const uint32_t cnt = min(kWalkChunk, cu.jend - cu.j);
+    uint32_t m = 0;
+    for (uint32_t u = cnt; u > 0; --u) {  // last candidate first: candidate k ends at bit k
+      const uint32_t j = cu.j + u - 1;
+      const float2 x = src.x[j + cu.tb];
+      const float dx = x.x - q.xi.x, dy = x.y - q.xi.y;
+      const float d2 = dx * dx + dy * dy;
+      m = (m << 1) | ((d2 < p.h2 && j != (uint32_t)i) ? 1u : 0u);
+    }
+    cu.mbase = cu.j;
+    cu.mask = m;
+    cu.j += cnt;
   }
-  return false;
 }
 
 // The pair's 16 BF16 features as 8 words (features 2k, 2k+1 in word k).
@@ -278,7 +293,7 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
 
   // 1. pair layer: H_i = sum_j relu(W1 f_ij)
   PartInfo q{};
-  Cursor cu{0u, 0u, 0u, 0, true};
+  Cursor cu{0u, 0u, 0u, 0, true, 0u, 0u};
   if (me < n) {
     q = part_info<MULTI>(b, p, s0, norm, me);
     cu.done = false;
@@ -592,7 +607,7 @@ __global__ __launch_bounds__(64) void k_sur_back_pairs(Buffers b, DevParams p, D
       d11[k] = r1 < n ? m.dH[(size_t)r1 * kHid + 32 + c] : 0.0f;
     }
     PartInfo q{};
-    Cursor cu{0u, 0u, 0u, 0, true};
+    Cursor cu{0u, 0u, 0u, 0, true, 0u, 0u};
     if (me < n) {
       q = part_info<MULTI>(b, p, s0, norm, me);
       cu.done = false;
