@@ -179,7 +179,7 @@ __device__ __forceinline__ PairSrc stage_pairs(const Buffers &b, const DevScene 
 // Advances to the lane's next pair (j != i, r^2 < h^2); false when exhausted.
 // t is the pair's index in src.  Candidates are tested kWalkChunk at a time
 // into a mask, whose bits are then taken lowest first (candidate order).
-constexpr uint32_t kWalkChunk = 8;
+constexpr uint32_t kWalkChunk = 16;
 __device__ __forceinline__ bool next_pair(Cursor &cu, const Buffers &b, const DevParams &p, const PairSrc &src,
                                           const PartInfo &q, int i, uint32_t &t, float2 &xj, float &Rx, float &Ry,
                                           float &r2) {
