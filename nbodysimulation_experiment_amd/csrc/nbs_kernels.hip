@@ -172,23 +172,22 @@ __device__ __forceinline__ uint32_t for_each_pair_wide(const uint32_t *start, co
 // atomics per wave instead of one per particle.  All active lanes must call it.
 __device__ __forceinline__ uint32_t wave_aggregated_fill(uint32_t *count, uint32_t key) {
   const int lane = (int)(threadIdx.x & 63u);
-  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t todo = __ballot(1);
   int leader = lane;
-  uint32_t rank = 0, size = 0;
-  while (todo) {
+  uint64_t group = 0;
+  while (todo) {  // one pass per distinct key; the key and masks are wave-uniform
     const int l = __builtin_ctzll(todo);
-    const uint32_t k = (uint32_t)__shfl((int)key, l, 64);
+    const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)key, l);
     const uint64_t m = __ballot(key == k) & todo;
-    if ((m >> lane) & 1ull) {
+    if (key == k) {
       leader = l;
-      rank = (uint32_t)__popcll(m & below);
-      size = (uint32_t)__popcll(m);
+      group = m;
     }
     todo &= ~m;
   }
+  const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(group >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)group, 0u));
   uint32_t base = 0;
-  if (leader == lane) base = atomicAdd(&count[key], size);
+  if (leader == lane) base = atomicAdd(&count[key], (uint32_t)__popcll(group));
   base = (uint32_t)__shfl((int)base, leader, 64);
   return base + rank;
 }
