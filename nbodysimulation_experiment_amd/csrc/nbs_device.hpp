@@ -266,19 +266,37 @@ struct WaveBox {
   int x0, y0, x1, y1;
 };
 
+// Minimum / maximum over the 16 lanes of each DPP row (quad swaps, half-row
+// and row mirrors), then over the four rows' lanes 0, 16, 32, 48 in scalars.
+__device__ __forceinline__ int row_min(int v) {
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
+  return v;
+}
+__device__ __forceinline__ int row_max(int v) {
+  v = max(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));
+  v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));
+  v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));
+  v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));
+  return v;
+}
+__device__ __forceinline__ int wave_min(int v) {
+  v = row_min(v);
+  return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+             min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+__device__ __forceinline__ int wave_max(int v) {
+  v = row_max(v);
+  return max(max(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+             max(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
 __device__ __forceinline__ WaveBox wave_box(float x, float y, bool live) {
   const int ox = float_order(__float_as_int(x)), oy = float_order(__float_as_int(y));
-  int x0 = live ? ox : INT_MAX, y0 = live ? oy : INT_MAX;
-  int x1 = live ? ox : INT_MIN, y1 = live ? oy : INT_MIN;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    x0 = min(x0, __shfl_xor(x0, o, 64));
-    y0 = min(y0, __shfl_xor(y0, o, 64));
-    x1 = max(x1, __shfl_xor(x1, o, 64));
-    y1 = max(y1, __shfl_xor(y1, o, 64));
-  }
-  return {__builtin_amdgcn_readfirstlane(x0), __builtin_amdgcn_readfirstlane(y0),
-          __builtin_amdgcn_readfirstlane(x1), __builtin_amdgcn_readfirstlane(y1)};
+  return {wave_min(live ? ox : INT_MAX), wave_min(live ? oy : INT_MAX), wave_max(live ? ox : INT_MIN),
+          wave_max(live ? oy : INT_MIN)};
 }
 
 // Bodies among [c0, c0 + cnt) (cnt <= 64) that may move a position inside
