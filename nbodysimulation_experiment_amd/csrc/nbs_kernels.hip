@@ -418,19 +418,47 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Buffers b, int n) {
   b.tmpId[pos] = b.id[s];
 }
 
-__global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
-  const int p = (int)(blockIdx.x * kBlock + threadIdx.x);
-  if (p >= n) return;
-  const uint32_t s = b.tmpSlot[p];
-  const uint32_t myId = b.tmpId[p];
-  const uint32_t key = b.keyp[s];
-  const uint32_t c0 = b.startNext[key], c1 = b.startNext[key + 1];
+// A workgroup's slots are consecutive in cell order, so the cells they belong
+// to span [c0 of its first slot, c1 of its last slot); when that span fits
+// kRankTile, its particle indices are staged in LDS for the ranking loops.
+constexpr uint32_t kRankTile = 2048;
+
+template <typename Ids>
+__device__ __forceinline__ uint32_t rank_in_cell(Ids ids, uint32_t c0, uint32_t c1, uint32_t myId) {
   uint32_t rank = 0, q = c0;
   for (; q + 4u <= c1; q += 4u) {  // four independent loads in flight
-    const uint32_t a0 = b.tmpId[q], a1 = b.tmpId[q + 1], a2 = b.tmpId[q + 2], a3 = b.tmpId[q + 3];
+    const uint32_t a0 = ids(q), a1 = ids(q + 1), a2 = ids(q + 2), a3 = ids(q + 3);
     rank += (a0 < myId ? 1u : 0u) + (a1 < myId ? 1u : 0u) + (a2 < myId ? 1u : 0u) + (a3 < myId ? 1u : 0u);
   }
-  for (; q < c1; ++q) rank += (b.tmpId[q] < myId) ? 1u : 0u;
+  for (; q < c1; ++q) rank += (ids(q) < myId) ? 1u : 0u;
+  return rank;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
+  __shared__ uint32_t tileIds[kRankTile];
+  __shared__ uint32_t span[2];
+  const int p0 = (int)(blockIdx.x * kBlock);
+  const int p = p0 + (int)threadIdx.x;
+  const bool live = p < n;
+  uint32_t s = 0, myId = 0, key = 0, c0 = 0, c1 = 0;
+  if (live) {
+    s = b.tmpSlot[p];
+    myId = b.tmpId[p];
+    key = b.keyp[s];
+    c0 = b.startNext[key];
+    c1 = b.startNext[key + 1];
+  }
+  if (p == p0) span[0] = c0;
+  if (p == min(p0 + kBlock, n) - 1) span[1] = c1;
+  __syncthreads();
+  const uint32_t lo = span[0], hi = span[1];
+  const bool tiled = hi - lo <= kRankTile;
+  if (tiled)
+    for (uint32_t q = lo + threadIdx.x; q < hi; q += kBlock) tileIds[q - lo] = b.tmpId[q];
+  __syncthreads();
+  if (!live) return;
+  const uint32_t rank = tiled ? rank_in_cell([&](uint32_t q) { return tileIds[q - lo]; }, c0, c1, myId)
+                              : rank_in_cell([&](uint32_t q) { return b.tmpId[q]; }, c0, c1, myId);
   const uint32_t d = c0 + rank;
   b.x1[d] = b.xp[s];
   b.prev[d] = b.x0[s];  // demo4.cpp:334 prevPosition = curPosition
