@@ -275,11 +275,21 @@ def main():
             t = json.load(open(tpath))
             if t.get("workload") == args.workload and dom in t.get("kernels", {}):
                 traffic = t["kernels"][dom]
+        # the kernel's actual limiter: VALU issue, from the instruction-counter profile
+        valu = None
+        vpath = os.path.join(ROOT, "profiles", "valu_issue.json")
+        if os.path.exists(vpath):
+            v = json.load(open(vpath))
+            if v.get("workload") == args.workload and dom in v.get("kernels", {}):
+                valu = v["kernels"][dom]
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
                     "kernel_avg_ms": round(avg_ms, 5), "alg_bytes_per_launch": alg,
                     "copy_ceiling_gbs": round(copy_gbs, 1),
-                    "step_alg_bytes_frac": round(value * STEP_ALG_BYTES / 1e9 / HBM_PEAK_GBS, 5)}
+                    "step_alg_bytes_frac": round(value * STEP_ALG_BYTES / 1e9 / HBM_PEAK_GBS, 5),
+                    "valu_issue_frac": valu,
+                    "valu_issue_source": "profiles/valu_issue.json (rocprofv3 SQ_INSTS_VALU / GRBM_GUI_ACTIVE)"
+                    if valu is not None else None}
 
     surrogate = None
     online = args.workload == "batched-surrogate"  # in-run training on each check step's plain frame

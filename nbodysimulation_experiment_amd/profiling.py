@@ -2,6 +2,7 @@
 
     python -m nbodysimulation_experiment_amd.profiling stats  <kernel_stats.csv>
     python -m nbodysimulation_experiment_amd.profiling counters <counter_collection.csv> [--copy-bytes N]
+                                                              [--workload W --json valu.json]
     python -m nbodysimulation_experiment_amd.profiling traffic <fetch.csv> <write.csv> --copy-bytes N
                                                               --workload W [--json out.json]
 
@@ -109,6 +110,32 @@ def traffic(fetch_csv: str, write_csv: str, copy_bytes: float, workload: str) ->
             "write_factor": round(wf, 4), "kernels": kernels, "detail": detail}
 
 
+CUS = 256  # MI355X compute units; GRBM_GUI_ACTIVE is summed over its 8 XCDs
+
+
+def valu_issue(c: dict, workload: str) -> dict:
+    """VALU issue utilisation of each kernel group from a counters() result:
+    SQ_INSTS_VALU / (CUS * GRBM_GUI_ACTIVE / 8), a CU issuing at most one wave64
+    VALU instruction per cycle; the single-scene LDS-tile variants pooled by
+    dispatches (multi-scene variants, "<t,N>", belong to other workloads)."""
+    per = collections.defaultdict(lambda: [0.0, 0.0])  # kernel: [instr, cycles] summed over dispatches
+    for k, v in c.items():
+        if "SQ_INSTS_VALU" not in v or "GRBM_GUI_ACTIVE" not in v or "<t," in k:
+            continue
+        name = k.split("<")[0]
+        per[name][0] += v["SQ_INSTS_VALU"] * v["dispatches"]
+        per[name][1] += v["GRBM_GUI_ACTIVE"] * v["dispatches"]
+    groups = {}
+    for g, variants in GROUPS.items():
+        for members in variants:  # the first alternative that ran, its kernels together
+            if all(k in per for k in members):
+                ins = sum(per[k][0] for k in members)
+                cyc = sum(per[k][1] for k in members)
+                groups[g] = round(ins / (CUS * cyc / 8.0), 4)
+                break
+    return {"workload": workload, "unit": "fraction of VALU issue slots", "kernels": groups}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("mode", choices=["stats", "counters", "traffic"])
@@ -142,6 +169,9 @@ def main(argv=None):
             print(f"{k + '<f,*>':32s} calls {calls:6d}  avg {1e3 * tot / calls:9.2f} us  total {tot:9.3f} ms  (pooled)")
         return
     c = counters(a.path)
+    if a.json:
+        with open(a.json, "w") as fh:
+            fh.write(json.dumps(valu_issue(c, a.workload), indent=1) + "\n")
     if a.copy_bytes and "copy" in c:
         for key in ("FETCH_SIZE", "WRITE_SIZE"):
             if key in c["copy"]:
