@@ -400,7 +400,7 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
     }
     // 3. o = W4 G + b4 (FP32): lane (c, g) computes output g of particle i
     float o = prm[kOffB4 + g];
-    for (int k = 0; k < kHid; ++k) o = o + prm[kOffW4 + g * kHid + k] * gbuf[c][k];
+    for (int k = 0; k < kHid; ++k) o = __builtin_fmaf(prm[kOffW4 + g * kHid + k], gbuf[c][k], o);
     const float oy = __shfl_down(o, 32, 64);
     const bool head = g == 0 && i < n;
     if (head && m.out) {
@@ -429,9 +429,10 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
 // ---------------------------------------------------------------------------
 // Backward
 // ---------------------------------------------------------------------------
-// Per particle (FP32): dO, dG (pre-activation), dM, dH, the squared error and
-// the squared normalised target.  The loops are unrolled so the per-particle
-// vectors stay in registers; the weights are wave-uniform (scalar) loads.
+// Per particle (FP32, fused multiply-adds): dO, dG (pre-activation), dM, dH,
+// the squared error and the squared normalised target.  The loops are unrolled
+// so the per-particle vectors stay in registers; the weights are wave-uniform
+// (scalar) loads.
 __global__ __launch_bounds__(kBlock) void k_sur_back_head(int n, SurrogateBufs m, float invN) {
   __shared__ float red[2][kBlock / 64];
   const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
@@ -453,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_sur_back_head(int n, SurrogateBufs m
     float vin[kHid], acc[kHid];
 #pragma clang loop unroll(full)
     for (int k = 0; k < kHid; ++k) {
-      const float v = prm[kOffW4 + k] * d0 + prm[kOffW4 + kHid + k] * d1;
+      const float v = __builtin_fmaf(prm[kOffW4 + k], d0, prm[kOffW4 + kHid + k] * d1);
       vin[k] = G[k] > 0.0f ? v : 0.0f;
       dG[k] = vin[k];
       acc[k] = 0.0f;
@@ -462,7 +463,7 @@ __global__ __launch_bounds__(kBlock) void k_sur_back_head(int n, SurrogateBufs m
 #pragma clang loop unroll(full)
     for (int r = 0; r < kHid; ++r) {
 #pragma clang loop unroll(full)
-      for (int k = 0; k < kHid; ++k) acc[k] += prm[kOffW3 + r * kHid + k] * vin[r];
+      for (int k = 0; k < kHid; ++k) acc[k] = __builtin_fmaf(prm[kOffW3 + r * kHid + k], vin[r], acc[k]);
     }
 #pragma clang loop unroll(full)
     for (int k = 0; k < kHid; ++k) {
@@ -473,7 +474,7 @@ __global__ __launch_bounds__(kBlock) void k_sur_back_head(int n, SurrogateBufs m
 #pragma clang loop unroll(full)
     for (int r = 0; r < kHid; ++r) {
 #pragma clang loop unroll(full)
-      for (int k = 0; k < kHid; ++k) acc[k] += prm[kOffW2 + r * kHid + k] * vin[r];
+      for (int k = 0; k < kHid; ++k) acc[k] = __builtin_fmaf(prm[kOffW2 + r * kHid + k], vin[r], acc[k]);
     }
 #pragma clang loop unroll(full)
     for (int k = 0; k < kHid; ++k) dH[k] = acc[k];
@@ -539,9 +540,9 @@ __global__ __launch_bounds__(kBlock) void k_outer_partial(int n, const float *A,
 #pragma unroll
       for (int x = 0; x < 4; ++x) {
         const float av = sa[rr][a0 + x];
-        accb[x] += av * ss[rr];
+        accb[x] = __builtin_fmaf(av, ss[rr], accb[x]);
 #pragma unroll
-        for (int y = 0; y < 4; ++y) acc[x][y] += av * sb[rr][b0 + y];
+        for (int y = 0; y < 4; ++y) acc[x][y] = __builtin_fmaf(av, sb[rr][b0 + y], acc[x][y]);
       }
     }
     __syncthreads();
