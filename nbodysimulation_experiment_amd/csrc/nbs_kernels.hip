@@ -1,10 +1,10 @@
 // HIP kernels of the SPH step for gfx950 (MI355X).
 //
-// One step is five launches on one stream:
+// One step is five kernel groups on one stream:
 //   1. k_visc_predict   integrate, viscosity (Jacobi gather over the previous
 //                       binning), predict, cell key of the predicted position and
 //                       the per-cell arrival count (demo4.cpp:301-356)
-//   2. k_scan_reduce + k_scan_final
+//   2. k_scan_reduce + k_scan_final (k_scan_small for small grids)
 //                       exclusive scan of the per-cell counts -> cell starts
 //   3. k_scatter + k_rank_reorder
 //                       bucket the particles by cell, order each cell by particle
@@ -13,6 +13,8 @@
 //   5. k_delta          double-density relaxation (Jacobi gather), plane /
 //                       circle / segment / polygon collisions, velocity
 //                       (demo4.cpp:239-255, 411-450)
+// Small steps use the wide variants of 1, 4 and 5 (one wave per particle;
+// 1 is then followed by k_count_fill).
 //
 // Every FP32 expression keeps the reference's operand order and the library
 // is compiled with -ffp-contract=off and correctly rounded sqrt / divide, so the
@@ -28,10 +30,9 @@ namespace nbs {
 // positions from X at idx(d, j) (the LDS tile or the slot array); returns the
 // number of candidates (the reference's neighbour count).  Each row range is
 // taken in chunks of 32: a cheap pass marks the in-kernel candidates in a
-// 32-bit mask (the chunk index t is the same in every lane, so its bit is a
-// scalar), then the pair work runs only for the marked ones, lowest first, so
-// the accumulation order is the candidate order and the expensive work is not
-// issued for candidates outside the kernel radius.
+// 32-bit mask, then the pair work runs only for the marked ones, lowest first,
+// so the accumulation order is the candidate order and the expensive work is
+// not issued for candidates outside the kernel radius.
 template <typename Idx, typename F>
 __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const DevScene &sc, int cx, int cy,
                                                   const float2 *X, Idx idx, float2 xi, float h2, F &&f) {
