@@ -200,14 +200,17 @@ __device__ __forceinline__ bool next_pair(Cursor &cu, const Buffers &b, const De
       return false;
     }
     const uint32_t cnt = min(kWalkChunk, cu.jend - cu.j);
+    // m = 2 * m + (d2 < h2), last candidate first, so candidate k ends at bit
+    // k: the compare's lane mask is the add's carry-in
     uint32_t m = 0;
-    for (uint32_t u = cnt; u > 0; --u) {  // last candidate first: candidate k ends at bit k
-      const uint32_t j = cu.j + u - 1;
-      const float2 x = src.x[j + cu.tb];
+    for (uint32_t u = cnt; u > 0; --u) {
+      const float2 x = src.x[cu.j + u - 1 + cu.tb];
       const float dx = x.x - q.xi.x, dy = x.y - q.xi.y;
       const float d2 = dx * dx + dy * dy;
-      m = (m << 1) | ((d2 < p.h2 && j != (uint32_t)i) ? 1u : 0u);
+      asm("v_cmp_gt_f32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc" : "+v"(m) : "s"(p.h2), "v"(d2) : "vcc");
     }
+    const uint32_t self = (uint32_t)i - cu.j;  // the particle itself is no pair
+    if (self < cnt) m &= ~(1u << self);
     cu.mbase = cu.j;
     cu.mask = m;
     cu.j += cnt;
