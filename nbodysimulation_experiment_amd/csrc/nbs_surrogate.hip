@@ -63,6 +63,11 @@ __device__ __forceinline__ bf16x4 load_wb(const uint16_t *W, int K, int n0, int 
   return words_to_bf4(w.x, w.y);
 }
 
+// relu(z) as an integer maximum of the bit pattern with 0: negative values
+// (and -0) become +0, others are unchanged.  One instruction; fmaxf costs two
+// here (the IEEE-mode canonicalisation of its operand).
+__device__ __forceinline__ float relu(float z) { return __int_as_float(max(__float_as_int(z), 0)); }
+
 // C-layout row of register r for lane group g.
 __device__ __forceinline__ int c_row(int r, int g) { return 8 * (r >> 2) + 4 * g + (r & 3); }
 
@@ -319,14 +324,14 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
     pair_layer(a.t0s0, a.t0s1, w1, zlo, zhi);
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      h00[k] += fmaxf(zlo[k], 0.0f);
-      h01[k] += fmaxf(zhi[k], 0.0f);
+      h00[k] += relu(zlo[k]);
+      h01[k] += relu(zhi[k]);
     }
     pair_layer(a.t1s0, a.t1s1, w1, zlo, zhi);
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      h10[k] += fmaxf(zlo[k], 0.0f);
-      h11[k] += fmaxf(zhi[k], 0.0f);
+      h10[k] += relu(zlo[k]);
+      h11[k] += relu(zhi[k]);
     }
   }
   shCnt[lane] = (float)np;
@@ -393,8 +398,8 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const int row = c_row(k, g);
-      gbuf[row][c] = fmaxf(olo[k] + prm[kOffB3 + c], 0.0f);
-      gbuf[row][32 + c] = fmaxf(ohi[k] + prm[kOffB3 + 32 + c], 0.0f);
+      gbuf[row][c] = relu(olo[k] + prm[kOffB3 + c]);
+      gbuf[row][32 + c] = relu(ohi[k] + prm[kOffB3 + 32 + c]);
     }
     __syncthreads();
     if (train && g == 0 && i < n) {
