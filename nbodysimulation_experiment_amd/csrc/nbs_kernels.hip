@@ -25,17 +25,20 @@
 
 namespace nbs {
 
-// Calls f(j, Rx, Ry, r2) for every candidate slot j of the particle at xi in
-// cell (cx, cy) whose r2 < h2 (sph.h:470, 488, 502), in candidate order, reading
-// positions from X at idx(d, j) (the LDS tile or the slot array); returns the
-// number of candidates (the reference's neighbour count).  Each row range is
-// taken in chunks of 32: a cheap pass marks the in-kernel candidates in a
-// 32-bit mask, then the pair work runs only for the marked ones, lowest first,
-// so the accumulation order is the candidate order and the expensive work is
-// not issued for candidates outside the kernel radius.
-template <typename Idx, typename F>
+// For every candidate slot j of the particle at xi in cell (cx, cy) whose
+// r2 < h2 (sph.h:470, 488, 502), in candidate order, applies the pair's
+// contribution: apply(term(j, Rx, Ry, r2, true)).  term(..., false) must return
+// a contribution that leaves the accumulators unchanged bit for bit.
+// Positions are read from X at idx(d, j) (the LDS tile or the slot array);
+// returns the number of candidates (the reference's neighbour count).  Each
+// row range is taken in chunks of 32: a cheap pass marks the in-kernel
+// candidates in a 32-bit mask, then the pair work runs only for the marked
+// ones, lowest first, so the accumulation order is the candidate order and
+// the expensive work is not issued for candidates outside the kernel radius.
+template <typename Idx, typename Term, typename Apply>
 __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const DevScene &sc, int cx, int cy,
-                                                  const float2 *X, Idx idx, float2 xi, float h2, F &&f) {
+                                                  const float2 *X, Idx idx, float2 xi, float h2, Term &&term,
+                                                  Apply &&apply) {
   uint32_t cand = 0;
   for (int d = 0; d < 3; ++d) {
     const RowRange rr = row_range(start, sc, cx, cy, d - 1);
@@ -66,12 +69,22 @@ __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const D
         push(e);
       }
       for (; u > 0u; --u) push(dist2(u - 1));
+      // Two pairs per iteration: their loads and arithmetic are independent,
+      // and the contributions are applied lowest first.  When only one marked
+      // bit is left the second slot repeats it and contributes the identity.
       while (mask) {
-        const uint32_t t = (uint32_t)__builtin_ctz(mask);
+        const uint32_t ta = (uint32_t)__builtin_ctz(mask);
         mask &= mask - 1u;
-        const float2 xj = X[q0 + t];
-        const float Rx = xj.x - xi.x, Ry = xj.y - xi.y;
-        f(base + t, Rx, Ry, Rx * Rx + Ry * Ry);
+        const bool two = mask != 0u;
+        const uint32_t tb = two ? (uint32_t)__builtin_ctz(mask) : ta;
+        mask &= two ? mask - 1u : ~0u;
+        const float2 xa = X[q0 + ta], xb = X[q0 + tb];
+        const float Rax = xa.x - xi.x, Ray = xa.y - xi.y;
+        const float Rbx = xb.x - xi.x, Rby = xb.y - xi.y;
+        const auto A = term(base + ta, Rax, Ray, Rax * Rax + Ray * Ray, true);
+        const auto B = term(base + tb, Rbx, Rby, Rbx * Rbx + Rby * Rby, two);
+        apply(A);
+        apply(B);
       }
     }
   }
@@ -202,24 +215,29 @@ template <typename Idx>
 __device__ __forceinline__ void viscosity_sum(const uint32_t *start, const DevScene &sc, int cx, int cy,
                                               const float2 *X, const float2 *V, Idx idx, const DevParams &p,
                                               float2 xi, float vx, float vy, float &accx, float &accy) {
-  for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t j, float Rx, float Ry, float r2) {
-    const float2 vj = V[j];
-    const float vjx = p.gdtx + vj.x, vjy = p.gdty + vj.y;
-    float r, inv;
-    pair_length(r2, r, inv);                              // sph.h:503, 505 Vec2Normalize (length 0 -> 1)
-    const float qq = r * p.invH;                          // sph.h:504
-    const float nX = Rx * inv, nY = Ry * inv;
-    const float u = (vx - vjx) * nX + (vy - vjy) * nY;    // sph.h:506
-    if (u > 0.0f) {
-      const float f = (1.0f - qq) * (p.sigma * u + p.beta * (u * u));  // sph.h:508
-      const float ix = ((f * nX) * 0.5f) * p.dt;         // demo4.cpp:233 force * 0.5f * deltaTime
-      const float iy = ((f * nY) * 0.5f) * p.dt;
-      accx = accx - ix;                                   // own visit
-      accy = accy - iy;
-      accx = accx - ix;                                   // neighbour's visit (f_ji = -f_ij)
-      accy = accy - iy;
-    }
-  });
+  for_each_pair(
+      start, sc, cx, cy, X, idx, xi, p.h2,
+      [&](uint32_t j, float Rx, float Ry, float r2, bool use) {
+        const float2 vj = V[j];
+        const float vjx = p.gdtx + vj.x, vjy = p.gdty + vj.y;
+        float r, inv;
+        pair_length(r2, r, inv);                          // sph.h:503, 505 Vec2Normalize (length 0 -> 1)
+        const float qq = r * p.invH;                      // sph.h:504
+        const float nX = Rx * inv, nY = Ry * inv;
+        const float u = (vx - vjx) * nX + (vy - vjy) * nY;  // sph.h:506
+        const float f = (1.0f - qq) * (p.sigma * u + p.beta * (u * u));  // sph.h:508
+        const float ix = ((f * nX) * 0.5f) * p.dt;       // demo4.cpp:233 force * 0.5f * deltaTime
+        const float iy = ((f * nY) * 0.5f) * p.dt;
+        // no impulse unless u > 0 (sph.h:507): subtracting +0 leaves the sums unchanged
+        const bool on = use && u > 0.0f;
+        return make_float2(on ? ix : 0.0f, on ? iy : 0.0f);
+      },
+      [&](float2 im) {
+        accx = accx - im.x;                               // own visit
+        accy = accy - im.y;
+        accx = accx - im.x;                               // neighbour's visit (f_ji = -f_ij)
+        accy = accy - im.y;
+      });
 }
 
 template <bool MULTI, int CAP>
@@ -526,12 +544,19 @@ template <typename Idx>
 __device__ __forceinline__ uint32_t density_sum(const uint32_t *start, const DevScene &sc, int cx, int cy,
                                                 const float2 *X, Idx idx, const DevParams &p, float2 xi, float &d0,
                                                 float &d1) {
-  return for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t, float, float, float r2) {
-    const float r = pair_sqrt(r2);                        // sph.h:471
-    const float term = 1.0f - r * p.invH;                 // sph.h:472
-    d0 = d0 + (term * term);                              // sph.h:473
-    d1 = d1 + ((term * term) * term);                     // sph.h:474
-  });
+  return for_each_pair(
+      start, sc, cx, cy, X, idx, xi, p.h2,
+      [&](uint32_t, float, float, float r2, bool use) {
+        const float r = pair_sqrt(r2);                    // sph.h:471
+        const float term = 1.0f - r * p.invH;             // sph.h:472
+        const float t2 = term * term, t3 = (term * term) * term;
+        // adding -0 leaves the sums unchanged
+        return make_float2(use ? t2 : -0.0f, use ? t3 : -0.0f);
+      },
+      [&](float2 w) {
+        d0 = d0 + w.x;                                    // sph.h:473
+        d1 = d1 + w.y;                                    // sph.h:474
+      });
 }
 
 // Plans the tile of the workgroup at slots [i0, i0 + kBlock) of the new binning.
@@ -590,20 +615,27 @@ template <typename Idx>
 __device__ __forceinline__ void delta_sum(const uint32_t *start, const DevScene &sc, int cx, int cy, const float2 *X,
                                           const float2 *P, Idx idx, const DevParams &p, float2 xi, float2 pi,
                                           float &accx, float &accy) {
-  for_each_pair(start, sc, cx, cy, X, idx, xi, p.h2, [&](uint32_t j, float Rx, float Ry, float r2) {
-    const float2 pj = P[j];
-    float r, inv;
-    pair_length(r2, r, inv);                              // sph.h:489-490 Vec2Normalize (length 0 -> 1)
-    const float nX = Rx * inv, nY = Ry * inv;
-    const float term = 1.0f - r * p.invH;                 // sph.h:491
-    const float tt = term * term;
-    const float dij = p.dt2 * (pi.x * term + pi.y * tt);  // sph.h:492 with P_i
-    const float dji = p.dt2 * (pj.x * term + pj.y * tt);  // sph.h:492 with P_j (j's visit)
-    accx = accx - ((dij * nX) * 0.5f);                    // demo4.cpp:251 dx -= delta * 0.5f
-    accy = accy - ((dij * nY) * 0.5f);
-    accx = accx - ((dji * nX) * 0.5f);                    // demo4.cpp:250 (j's visit)
-    accy = accy - ((dji * nY) * 0.5f);
-  });
+  for_each_pair(
+      start, sc, cx, cy, X, idx, xi, p.h2,
+      [&](uint32_t j, float Rx, float Ry, float r2, bool use) {
+        const float2 pj = P[j];
+        float r, inv;
+        pair_length(r2, r, inv);                          // sph.h:489-490 Vec2Normalize (length 0 -> 1)
+        const float nX = Rx * inv, nY = Ry * inv;
+        const float term = 1.0f - r * p.invH;             // sph.h:491
+        const float tt = term * term;
+        const float dij = p.dt2 * (pi.x * term + pi.y * tt);  // sph.h:492 with P_i
+        const float dji = p.dt2 * (pj.x * term + pj.y * tt);  // sph.h:492 with P_j (j's visit)
+        // subtracting +0 leaves the sums unchanged
+        return make_float4(use ? (dij * nX) * 0.5f : 0.0f, use ? (dij * nY) * 0.5f : 0.0f,
+                           use ? (dji * nX) * 0.5f : 0.0f, use ? (dji * nY) * 0.5f : 0.0f);
+      },
+      [&](float4 d) {
+        accx = accx - d.x;                                // demo4.cpp:251 dx -= delta * 0.5f
+        accy = accy - d.y;
+        accx = accx - d.z;                                // demo4.cpp:250 (j's visit)
+        accy = accy - d.w;
+      });
 }
 
 template <bool MULTI, int CAP>
