@@ -35,6 +35,8 @@ namespace nbs {
 // candidates in a 32-bit mask, then the pair work runs only for the marked
 // ones, lowest first, so the accumulation order is the candidate order and
 // the expensive work is not issued for candidates outside the kernel radius.
+constexpr int kPairsPerIter = 4;
+
 template <typename Idx, typename Term, typename Apply>
 __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const DevScene &sc, int cx, int cy,
                                                   const float2 *X, Idx idx, float2 xi, float h2, Term &&term,
@@ -61,6 +63,13 @@ __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const D
         return Rx * Rx + Ry * Ry;
       };
       uint32_t u = cnt;
+      for (; u >= 8u; u -= 8u) {  // eight position loads in flight
+        float r2[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) r2[k] = dist2(u - 1 - k);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) push(r2[k]);
+      }
       for (; u >= 4u; u -= 4u) {
         const float a = dist2(u - 1), b = dist2(u - 2), c = dist2(u - 3), e = dist2(u - 4);
         push(a);
@@ -69,22 +78,34 @@ __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const D
         push(e);
       }
       for (; u > 0u; --u) push(dist2(u - 1));
-      // Two pairs per iteration: their loads and arithmetic are independent,
-      // and the contributions are applied lowest first.  When only one marked
-      // bit is left the second slot repeats it and contributes the identity.
+      // Four pairs per iteration: their loads and arithmetic are independent
+      // (a crowded particle's walk is a long serial chain, and this is what
+      // keeps it short), and the contributions are applied lowest first.
+      // Slots past the last marked bit repeat the first pair and contribute
+      // the identity.
       while (mask) {
-        const uint32_t ta = (uint32_t)__builtin_ctz(mask);
+        uint32_t t[kPairsPerIter];
+        bool use[kPairsPerIter];
+        t[0] = (uint32_t)__builtin_ctz(mask);
         mask &= mask - 1u;
-        const bool two = mask != 0u;
-        const uint32_t tb = two ? (uint32_t)__builtin_ctz(mask) : ta;
-        mask &= two ? mask - 1u : ~0u;
-        const float2 xa = X[q0 + ta], xb = X[q0 + tb];
-        const float Rax = xa.x - xi.x, Ray = xa.y - xi.y;
-        const float Rbx = xb.x - xi.x, Rby = xb.y - xi.y;
-        const auto A = term(base + ta, Rax, Ray, Rax * Rax + Ray * Ray, true);
-        const auto B = term(base + tb, Rbx, Rby, Rbx * Rbx + Rby * Rby, two);
-        apply(A);
-        apply(B);
+        use[0] = true;
+#pragma unroll
+        for (int k = 1; k < kPairsPerIter; ++k) {
+          use[k] = mask != 0u;
+          t[k] = use[k] ? (uint32_t)__builtin_ctz(mask) : t[0];
+          mask &= use[k] ? mask - 1u : ~0u;
+        }
+        float2 xs[kPairsPerIter];
+#pragma unroll
+        for (int k = 0; k < kPairsPerIter; ++k) xs[k] = X[q0 + t[k]];
+        decltype(term(0u, 0.0f, 0.0f, 0.0f, true)) c[kPairsPerIter];
+#pragma unroll
+        for (int k = 0; k < kPairsPerIter; ++k) {
+          const float Rx = xs[k].x - xi.x, Ry = xs[k].y - xi.y;
+          c[k] = term(base + t[k], Rx, Ry, Rx * Rx + Ry * Ry, use[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kPairsPerIter; ++k) apply(c[k]);
       }
     }
   }
