@@ -5,10 +5,15 @@
                                                               [--workload W --json valu.json]
     python -m nbodysimulation_experiment_amd.profiling traffic <fetch.csv> <write.csv> --copy-bytes N
                                                               --workload W [--json out.json]
+    python -m nbodysimulation_experiment_amd.profiling steps <counter_collection.csv> [--ranges 2,100,300,400,502]
 
 ``traffic`` turns one FETCH_SIZE run and one WRITE_SIZE run into bytes per
 launch of each kernel group (dispatches of every LDS-tile variant pooled).
-``counters`` averages each counter per dispatch of each kernel.  With
+``counters`` averages each counter per dispatch of each kernel.  ``steps``
+sums the counters of the pair kernels over ranges of steps (a step starts at
+each viscosity launch) and prints, per range, the VALU issue utilisation, the
+active-lane fraction of the VALU instructions (SQ_THREAD_CYCLES_VALU over 64
+lanes x SQ_INSTS_VALU) and, when collected, the waves resident per busy cycle.  With
 --copy-bytes, FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are converted to
 bytes with the factor that maps the copy kernel's reading onto its known
 traffic (N bytes read and N bytes written per dispatch).
@@ -136,15 +141,57 @@ def valu_issue(c: dict, workload: str) -> dict:
     return {"workload": workload, "unit": "fraction of VALU issue slots", "kernels": groups}
 
 
+def step_ranges(path: str, bounds: list[int]) -> list[dict]:
+    """Counters of the viscosity, density and relaxation kernels summed over
+    the step ranges [bounds[i], bounds[i + 1]), steps counted by viscosity
+    launches (the first launch is step 0)."""
+    per = collections.defaultdict(dict)
+    names = {}
+    for r in csv.DictReader(open(path)):
+        k = int(r["Dispatch_Id"])
+        per[k][r["Counter_Name"]] = float(r["Counter_Value"])
+        names[k] = short_name(r.get("Kernel_Name", r.get("Kernel-Name", ""))).split("<")[0]
+    step, acc = -1, collections.defaultdict(collections.Counter)
+    launches = collections.Counter()
+    for k in sorted(per):
+        n = names[k]
+        if n == "viscosity_predict":
+            step += 1
+        if n not in ("viscosity_predict", "density", "delta") or step < 0:
+            continue
+        for i in range(len(bounds) - 1):
+            if bounds[i] <= step < bounds[i + 1]:
+                acc[(i, n)].update(per[k])
+                launches[(i, n)] += 1
+    out = []
+    for (i, n), c in sorted(acc.items()):
+        row = {"steps": [bounds[i], bounds[i + 1]], "kernel": n, "launches": launches[(i, n)]}
+        if c.get("SQ_INSTS_VALU") and c.get("GRBM_GUI_ACTIVE"):
+            row["valu_issue"] = round(c["SQ_INSTS_VALU"] / (CUS * c["GRBM_GUI_ACTIVE"] / 8.0), 3)
+            row["valu_instr_per_launch_M"] = round(c["SQ_INSTS_VALU"] / launches[(i, n)] / 1e6, 2)
+            row["cycles_per_launch_k"] = round(c["GRBM_GUI_ACTIVE"] / 8.0 / launches[(i, n)] / 1e3, 1)
+        if c.get("SQ_THREAD_CYCLES_VALU") and c.get("SQ_INSTS_VALU"):
+            row["active_lanes"] = round(c["SQ_THREAD_CYCLES_VALU"] / (64.0 * c["SQ_INSTS_VALU"]), 3)
+        if c.get("SQ_WAVE_CYCLES") and c.get("SQ_BUSY_CYCLES"):
+            row["wave_cycles_per_busy_cycle"] = round(c["SQ_WAVE_CYCLES"] / c["SQ_BUSY_CYCLES"], 2)
+        out.append(row)
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["stats", "counters", "traffic"])
+    ap.add_argument("mode", choices=["stats", "counters", "traffic", "steps"])
     ap.add_argument("path")
     ap.add_argument("path2", nargs="?")
     ap.add_argument("--copy-bytes", type=float, default=0.0)
     ap.add_argument("--workload", default="dam-break-1m")
     ap.add_argument("--json", default="")
+    ap.add_argument("--ranges", default="2,100,300,400,502")
     a = ap.parse_args(argv)
+    if a.mode == "steps":
+        for row in step_ranges(a.path, [int(x) for x in a.ranges.split(",")]):
+            print(json.dumps(row))
+        return
     if a.mode == "traffic":
         t = traffic(a.path, a.path2, a.copy_bytes, a.workload)
         text = json.dumps(t, indent=1)
