@@ -7,7 +7,7 @@
 namespace nbs {
 
 constexpr int kBlock = 256;          // threads per workgroup of the particle kernels
-constexpr int kScanItems = 8;        // cells per thread in the scan kernels
+constexpr int kScanItems = 4;        // cells per thread in the scan kernels (1,024 per workgroup)
 constexpr int kScanChunk = kBlock * kScanItems;
 // One-workgroup cell scan for small grids (k_scan_small).
 constexpr int kSmallGridBlock = 1024;
