@@ -78,11 +78,10 @@ __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const D
         push(e);
       }
       for (; u > 0u; --u) push(dist2(u - 1));
-      // Four pairs per iteration: their loads and arithmetic are independent
-      // (a crowded particle's walk is a long serial chain, and this is what
-      // keeps it short), and the contributions are applied lowest first.
-      // Slots past the last marked bit repeat the first pair and contribute
-      // the identity.
+      // Four pairs per iteration: their loads and arithmetic are independent,
+      // which shortens the serial chain of a crowded particle's walk; the
+      // contributions are applied lowest first.  Slots past the last marked
+      // bit repeat the first pair and contribute the identity.
       while (mask) {
         uint32_t t[kPairsPerIter];
         bool use[kPairsPerIter];
