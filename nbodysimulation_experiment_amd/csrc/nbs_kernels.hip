@@ -14,7 +14,8 @@
 //                       circle / segment / polygon collisions, velocity
 //                       (demo4.cpp:239-255, 411-450)
 // Small steps use the wide variants of 1, 4 and 5 (one wave per particle;
-// 1 is then followed by k_count_fill).
+// 1 is then followed by k_count_fill, or for the smallest steps the counting
+// moves into the one-workgroup scan, k_count_scan_small).
 //
 // Every FP32 expression keeps the reference's operand order and the library
 // is compiled with -ffp-contract=off and correctly rounded sqrt / divide, so the
@@ -509,7 +510,10 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
 // kSmallGridCells cells: one launch instead of k_scan_reduce + k_scan_final,
 // with the same cell starts, count reset and occupied-cell statistics
 // (in cellStats[0..1]).
-__global__ __launch_bounds__(kSmallGridBlock) void k_scan_small(Buffers b, uint32_t cells) {
+// The one-workgroup scan body: counts from LDS (hist, when the workgroup has
+// just counted there) or from the global counters (reset for the next step).
+template <bool LDS_COUNTS>
+__device__ __forceinline__ void scan_small_body(Buffers &b, uint32_t cells, const uint32_t *hist) {
   constexpr int kWaves = kSmallGridBlock / 64;
   constexpr uint32_t kItems = kSmallGridCells / kSmallGridBlock;  // consecutive cells per thread
   __shared__ uint32_t waveTot[kWaves], waveMin[kWaves], waveMax[kWaves];
@@ -519,7 +523,7 @@ __global__ __launch_bounds__(kSmallGridBlock) void k_scan_small(Buffers b, uint3
   uint32_t sum = 0, mn = 0xFFFFFFFFu, mx = 0u;
 #pragma unroll
   for (uint32_t k = 0; k < kItems; ++k) {
-    v[k] = (c0 + k < cells) ? b.count[c0 + k] : 0u;
+    v[k] = (c0 + k < cells) ? (LDS_COUNTS ? hist[c0 + k] : b.count[c0 + k]) : 0u;
     sum += v[k];
     mx = max(mx, v[k]);
     if (v[k] > 0u) mn = min(mn, v[k]);
@@ -545,7 +549,7 @@ __global__ __launch_bounds__(kSmallGridBlock) void k_scan_small(Buffers b, uint3
   for (uint32_t k = 0; k < kItems; ++k) {
     if (c0 + k < cells) {
       b.startNext[c0 + k] = run;
-      b.count[c0 + k] = 0u;  // ready for the next step's counting
+      if (!LDS_COUNTS) b.count[c0 + k] = 0u;  // ready for the next step's counting
     }
     run += v[k];
   }
@@ -555,6 +559,26 @@ __global__ __launch_bounds__(kSmallGridBlock) void k_scan_small(Buffers b, uint3
     b.cellStats[1] = mx;
     b.startNext[cells] = tot;
   }
+}
+
+__global__ __launch_bounds__(kSmallGridBlock) void k_scan_small(Buffers b, uint32_t cells) {
+  scan_small_body<false>(b, cells, nullptr);
+}
+
+// Counting and scan of a small step in one workgroup: the cell counts are an
+// LDS histogram (kSmallGridCells counters), each particle's arrival index in
+// its cell is the LDS atomic's old value, then the scan runs from LDS.  Used
+// after the wide viscosity kernel (which leaves the counting to a later
+// kernel) for steps of at most kSmallCountParticles particles: one launch
+// instead of k_count_fill + k_scan_small, and no global atomics.
+__global__ __launch_bounds__(kSmallGridBlock) void k_count_scan_small(Buffers b, int n, uint32_t cells) {
+  __shared__ uint32_t hist[kSmallGridCells];
+  for (uint32_t c = threadIdx.x; c < cells; c += kSmallGridBlock) hist[c] = 0u;
+  __syncthreads();
+  for (int i = (int)threadIdx.x; i < n; i += kSmallGridBlock)
+    b.fill[i] = atomicAdd(&hist[b.keyp[i]], 1u);  // arrival slot; the rank pass orders the cell
+  __syncthreads();
+  scan_small_body<true>(b, cells, hist);
 }
 
 // ---------------------------------------------------------------------------
@@ -1031,8 +1055,12 @@ static unsigned wide_grid(int n) { return (unsigned)((n + kWidePerBlock - 1) / k
 
 uint32_t density_stat_slots(int n, int cap) { return cap == kWideCap ? wide_grid(n) : grid_for(n); }
 
+bool count_in_grid(int n, uint32_t cells, int viscCap) {
+  return viscCap == kWideCap && cells <= kSmallGridCells && n <= kSmallCountParticles;
+}
+
 void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int nBinned, int lists,
-                         int cap, hipStream_t s) {
+                         int cap, uint32_t cells, hipStream_t s) {
   if (n <= 0) return;
   if (cap == kWideCap) {
     if (b.nScenes > 1)
@@ -1040,7 +1068,7 @@ void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s
     else
       hipLaunchKernelGGL(k_visc_predict_wide<false>, dim3(wide_grid(n)), dim3(kBlock), 0, s, b, p, s0, n, nBinned,
                          lists);
-    hipLaunchKernelGGL(k_count_fill, dim3(grid_for(n)), dim3(kBlock), 0, s, b, n);
+    if (!count_in_grid(n, cells, cap)) hipLaunchKernelGGL(k_count_fill, dim3(grid_for(n)), dim3(kBlock), 0, s, b, n);
     return;
   }
   switch (cap) {
@@ -1063,9 +1091,11 @@ void launch_scatter_reorder(const Buffers &b, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_rank_reorder, dim3(grid_for(n)), dim3(kBlock), 0, s, b, n);
 }
 
-uint32_t launch_grid(const Buffers &b, uint32_t cells, int n, hipStream_t s) {
+uint32_t launch_grid(const Buffers &b, uint32_t cells, int n, int viscCap, hipStream_t s) {
   uint32_t slots = 1u;
-  if (cells <= kSmallGridCells) {
+  if (count_in_grid(n, cells, viscCap)) {
+    hipLaunchKernelGGL(k_count_scan_small, dim3(1), dim3(kSmallGridBlock), 0, s, b, n, cells);
+  } else if (cells <= kSmallGridCells) {
     hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kSmallGridBlock), 0, s, b, cells);
   } else {
     launch_scan(b, cells, s);

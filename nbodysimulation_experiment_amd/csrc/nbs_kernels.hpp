@@ -12,6 +12,7 @@ constexpr int kScanChunk = kBlock * kScanItems;
 // One-workgroup cell scan for small grids (k_scan_small).
 constexpr int kSmallGridBlock = 1024;
 constexpr uint32_t kSmallGridCells = 16384;
+constexpr int kSmallCountParticles = 16384;  // most particles counted by the one-workgroup scan
 
 // Step constants, derived on the host in FP32 exactly as the reference derives
 // them (sph.h:92 invKernelHeight, sph.h:470 h*h, sph.h:492 dt*dt,
@@ -89,14 +90,21 @@ void launch_unbin(const Buffers &b, int nBinned, hipStream_t stream);
 // cap: particle positions staged in LDS per workgroup (0, 1024, 2048 or 4096;
 // 0 reads neighbours from global memory).  Every choice gives bit-identical
 // results.
+// cells: the grid's cell count (the counting may move into the grid phase,
+// count_in_grid).
 void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int nBinned, int lists,
-                         int cap, hipStream_t stream);
+                         int cap, uint32_t cells, hipStream_t stream);
+// Whether the cell counting of a step runs in the grid phase's one-workgroup
+// scan (LDS histogram) instead of after the viscosity kernel: wide viscosity
+// (viscCap == kWideCap), at most kSmallGridCells cells and
+// kSmallCountParticles particles.
+bool count_in_grid(int n, uint32_t cells, int viscCap);
 void launch_scan(const Buffers &b, uint32_t cells, hipStream_t stream);
 void launch_scatter_reorder(const Buffers &b, int n, hipStream_t stream);
 // The grid phase: the cell scan (one workgroup for small grids, else
 // launch_scan), then launch_scatter_reorder.  Returns the number of (min, max)
 // occupied-cell pairs it leaves in cellStats.
-uint32_t launch_grid(const Buffers &b, uint32_t cells, int n, hipStream_t stream);
+uint32_t launch_grid(const Buffers &b, uint32_t cells, int n, int viscCap, hipStream_t stream);
 void launch_density(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int cap, hipStream_t stream);
 // Workgroups of the density launch for n particles at capacity cap: the number
 // of (min, max) candidate-count pairs it leaves in candStats.

@@ -862,9 +862,9 @@ class Simulation {
     int cap[3];
     pairCaps(cap);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 0], stream_));
-    launch_visc_predict(buf_, p, s0, n, (int)nBinned_, lists_ ? 1 : 0, cap[0], stream_);
+    launch_visc_predict(buf_, p, s0, n, (int)nBinned_, lists_ ? 1 : 0, cap[0], totalCells_, stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 1], stream_));
-    cellSlots_ = launch_grid(buf_, totalCells_, n, stream_);
+    cellSlots_ = launch_grid(buf_, totalCells_, n, cap[0], stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 2], stream_));
     launch_density(buf_, p, s0, n, cap[1], stream_);
     statSlots_ = density_stat_slots(n, cap[1]);
