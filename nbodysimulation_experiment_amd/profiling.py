@@ -30,7 +30,8 @@ import sys
 SHORT = [
     ("k_visc_predict_wide", "viscosity_predict_wide"), ("k_density_wide", "density_wide"),
     ("k_delta_wide", "delta_wide"),
-    ("k_visc_predict", "viscosity_predict"), ("k_count_fill", "count_fill"), ("k_scan_reduce", "scan_reduce"),
+    ("k_visc_predict", "viscosity_predict"), ("k_count_fill", "count_fill"),
+    ("k_count_scan_small", "count_scan_small"), ("k_scan_reduce", "scan_reduce"),
     ("k_scan_small", "scan_small"),
     ("k_scan_final", "scan_final"), ("k_scatter_grad", "scatter_grad"), ("k_scatter", "scatter"), ("k_rank_reorder", "rank_reorder"),
     ("k_density", "density"), ("k_delta", "delta"), ("k_copy", "copy"), ("k_sur_forward", "sur_forward"),
@@ -41,8 +42,9 @@ SHORT = [
 # group lists its alternatives: the kernels one step launches for the phase in
 # the one-lane-per-particle and in the wide form.
 GROUPS = {
-    "viscosity_predict": (("viscosity_predict",), ("viscosity_predict_wide", "count_fill")),
-    "grid": (("scan_reduce", "scan_final", "scatter", "rank_reorder"), ("scan_small", "scatter", "rank_reorder")),
+    "viscosity_predict": (("viscosity_predict",), ("viscosity_predict_wide", "count_fill"), ("viscosity_predict_wide",)),
+    "grid": (("scan_reduce", "scan_final", "scatter", "rank_reorder"), ("scan_small", "scatter", "rank_reorder"),
+             ("count_scan_small", "scatter", "rank_reorder")),
     "density": (("density",), ("density_wide",)),
     "delta": (("delta",), ("delta_wide",)),
     "surrogate_forward": (("sur_forward",),),
