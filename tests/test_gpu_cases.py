@@ -233,3 +233,24 @@ def test_dense_cells_bit_exact(caps):
     ref.step(DT, 12)
     assert_bit_exact(sim.get_state(), ref.state(), f"dense, caps {caps}: ")
     assert sim.GetStats()["maxParticleNeighborCount"] > 16 * 32
+
+
+@pytest.mark.parametrize("caps", [(4096, 4096, 4096), (0, 2048, 0), (-1, -1, -1)])
+def test_crowded_cells_bit_exact(caps):
+    """Particles at 0.05h spacing put about 400 in a cell: rows of 1,000+
+    candidates, dozens of 32-candidate chunks per row, and chunks with every
+    count of marked pairs, so the walk's four-pairs-per-iteration slots run
+    full, partly used and past the last marked bit.  1,156 particles leave a
+    ragged last workgroup; a second scene runs the multi-scene kernels.
+    Against the oracle bit for bit."""
+    pos = volume_positions((-0.8, -0.8), 34, 34, 0.05, 1.0, seed=41)
+    a = SceneSpec(20.0, 20.0, 1.0, pos, planes=wall_planes(20.0, 20.0))
+    b = SceneSpec(16.0, 16.0, 1.0, volume_positions((-0.5, -0.5), 20, 21, 0.05, 1.0, seed=42),
+                  planes=wall_planes(16.0, 16.0))
+    for specs in ([a], [a, b]):
+        sim, ref = gpu_for(specs), oracle_for(specs)
+        sim.set_tuning(autotune=False, tile_cap=caps, wide_max_particles=0)
+        sim.Update(DT, 8)
+        ref.step(DT, 8)
+        assert_bit_exact(sim.get_state(), ref.state(), f"crowded, {len(specs)} scene(s), caps {caps}: ")
+        assert sim.GetStats()["maxParticleNeighborCount"] > 1000
