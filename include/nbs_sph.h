@@ -206,7 +206,7 @@ nbs_status nbs_set_options(nbs_sim *sim, uint32_t flags);
  * workgroup's candidate rows in LDS tiles of tile_cap particles (0: no staging,
  * read neighbours from HBM/L2).  With autotune on, the library times every capacity on real steps
  * (all capacities give bit-identical results) and keeps the fastest per
- * kernel, again every retune_interval steps (0: once; default 64); steps of
+ * kernel, again every retune_interval steps (0: once; default 256); steps of
  * fewer than 1,024 particles use the wide kernels.  No reference
  * counterpart: the reference's only tunable is its thread count. */
 typedef struct nbs_tuning {

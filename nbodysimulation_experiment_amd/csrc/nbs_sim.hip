@@ -1338,7 +1338,7 @@ class Simulation {
   double kernelMs_[NBS_KERNEL_COUNT] = {};
   uint64_t kernelLaunches_[NBS_KERNEL_COUNT] = {};
 
-  nbs_tuning tuning_{1, 64, {2048, 2048, 2048}, 0};
+  nbs_tuning tuning_{1, 256, {2048, 2048, 2048}, 0};
   bool tuned_ = false;
   bool tuneActive_ = false;
   uint32_t statSlots_ = 0;  // candStats pairs written by the last step's density launch

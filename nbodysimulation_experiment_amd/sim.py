@@ -284,7 +284,7 @@ class ParticleSimulation:
         return {"autotune": bool(t.autotune), "retune_interval": int(t.retune_interval),
                 "tile_cap": [int(c) for c in t.tile_cap], "wide_max_particles": int(t.wide_max_particles)}
 
-    def set_tuning(self, autotune: bool = True, retune_interval: int = 64, tile_cap=(2048, 2048, 2048),
+    def set_tuning(self, autotune: bool = True, retune_interval: int = 256, tile_cap=(2048, 2048, 2048),
                    wide_max_particles: int = 0):
         t = NbsTuning(1 if autotune else 0, int(retune_interval), (ctypes.c_int32 * 3)(*tile_cap),
                       int(wide_max_particles))
