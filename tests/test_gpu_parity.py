@@ -103,9 +103,15 @@ def test_small_scene_bit_exact(steps):
     assert_bit_exact(sim.get_state(), ref.state(), f"after {steps} steps: ")
 
 
-def test_dam_break_4k_bit_exact_and_grid():
+@pytest.mark.parametrize("caps", [None, (-1, -1, -1)])
+def test_dam_break_4k_bit_exact_and_grid(caps):
+    """State, grid (cell starts and per-cell order) and statistics against the
+    oracle, with the autotuner's choices and with the wide kernels, whose
+    steps count cells in the one-workgroup scan's LDS histogram."""
     spec = dam_break_4k()
     sim = gpu_for([spec])
+    if caps is not None:
+        sim.set_tuning(autotune=False, tile_cap=caps, wide_max_particles=0)
     ref = oracle_for([spec])
     done = 0
     for target in (1, 10, 100, 250):
