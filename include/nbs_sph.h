@@ -210,7 +210,9 @@ nbs_status nbs_set_options(nbs_sim *sim, uint32_t flags);
  * fewer than 1,024 particles use the wide kernels.  No reference
  * counterpart: the reference's only tunable is its thread count. */
 typedef struct nbs_tuning {
-  int32_t autotune;
+  int32_t autotune;          /* 0: tile_cap fixed; 1: tune now, then every retune_interval steps;
+                                2: as 1, with tile_cap taken as an earlier tuning's result (no first tuning;
+                                   nbs_get_tuning then reports 1) */
   uint32_t retune_interval;
   int32_t tile_cap[3]; /* viscosity, density, relaxation: LDS tile capacity 0, 1024, 2048 or 4096, or
                           -1 for the wide kernels (one wave per particle); all give identical results */

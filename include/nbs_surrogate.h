@@ -25,9 +25,10 @@
  *       computed as well from the same input, the density of both results is
  *       evaluated on the step's grid, and the density error
  *       sum_i |rho_i(surrogate) - rho_i(plain)| / sum_i rho_i(plain)
- *       (FP64) is compared with max_density_error.  Above it the plain
- *       result replaces the surrogate's for that step and the next
- *       check_interval steps run the plain method (fallback).
+ *       (FP64), in total and per scene, is compared with
+ *       max_density_error.  If any scene's error is above it, the plain
+ *       result replaces the surrogate's for that step (in every scene) and
+ *       the next check_interval steps run the plain method (fallback).
  */
 #ifndef NBS_SURROGATE_H
 #define NBS_SURROGATE_H
@@ -47,12 +48,14 @@ typedef struct nbs_surrogate_report {
   uint64_t surrogate_steps;   /* steps relaxed by the network (check steps that passed included) */
   uint64_t plain_steps;       /* steps relaxed by the plain method while in surrogate mode */
   uint64_t check_steps;
-  uint64_t fallbacks;         /* checks above max_density_error */
+  uint64_t fallbacks;         /* checks with a scene above max_density_error */
   uint64_t train_steps;       /* Adam updates since nbs_surrogate_init */
   double last_density_error;
   double max_density_error;
   double mean_density_error;  /* over the checks since the last nbs_surrogate_set_mode */
   uint64_t last_pair_count;   /* in-range pairs the last network evaluation processed */
+  double last_max_scene_error;     /* largest per-scene density error of the last check */
+  uint64_t last_scenes_over_limit; /* scenes above max_density_error at the last check */
 } nbs_surrogate_report;
 
 /* Seeded initial weights (He-uniform, zero biases) and neutral normalisation;

@@ -73,7 +73,8 @@ class NbsSurrogateReport(ctypes.Structure):
     _fields_ = [(n, c_uint64) for n in ("surrogate_steps", "plain_steps", "check_steps", "fallbacks",
                                         "train_steps")] + \
                [(n, c_double) for n in ("last_density_error", "max_density_error", "mean_density_error")] + \
-               [("last_pair_count", c_uint64)]
+               [("last_pair_count", c_uint64), ("last_max_scene_error", c_double),
+                ("last_scenes_over_limit", c_uint64)]
 
 
 class NbsError(RuntimeError):

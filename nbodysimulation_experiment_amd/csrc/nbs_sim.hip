@@ -447,8 +447,12 @@ class Simulation {
           t.tile_cap[k] != kWideCap)
         throw Error(NBS_ERR_INVALID_ARGUMENT, "tile capacities are 0, 1024, 2048, 4096 or -1 (wide)");
     if (t.wide_max_particles < 0) throw Error(NBS_ERR_INVALID_ARGUMENT, "wide_max_particles must be >= 0");
+    if (t.autotune < 0 || t.autotune > 2) throw Error(NBS_ERR_INVALID_ARGUMENT, "autotune is 0, 1 or 2");
     tuning_ = t;
-    tuned_ = false;
+    // 2: tile_cap holds an earlier tuning's result; only the periodic
+    // re-tuning runs.
+    tuned_ = t.autotune == 2;
+    if (tuning_.autotune == 2) tuning_.autotune = 1;
     tuneActive_ = false;
     tunePending_.clear();
     stepsSinceTune_ = 0;
@@ -521,6 +525,8 @@ class Simulation {
     S.plainLeft = 0;
     S.rep.surrogate_steps = S.rep.plain_steps = S.rep.check_steps = S.rep.fallbacks = 0;
     S.rep.last_density_error = S.rep.max_density_error = S.rep.mean_density_error = 0.0;
+    S.rep.last_max_scene_error = 0.0;
+    S.rep.last_scenes_over_limit = 0;
     S.errSum = 0.0;
     destroyGraph();
   }
@@ -1246,17 +1252,32 @@ class Simulation {
     ++S.rep.surrogate_steps;
     return RELAX_SURROGATE;
   }
+  // The check passes only if every scene's density error is within the limit:
+  // the global error is the plain-density-weighted mean of the scene errors,
+  // so a small scene far off would pass a global test.
   void finishCheck() {
     SurrogateState &S = *sur_;
-    double e[2];
-    NBS_HIP(hipMemcpyAsync(e, S.errOut.p, sizeof(e), hipMemcpyDeviceToHost, stream_));
+    const size_t ns = scenes_.size();
+    std::vector<double> e(2 + 2 * ns);
+    NBS_HIP(hipMemcpyAsync(e.data(), S.errOut.p, 2 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    NBS_HIP(hipMemcpyAsync(e.data() + 2, S.sceneErr.p, 2 * ns * sizeof(double), hipMemcpyDeviceToHost, stream_));
     NBS_HIP(hipStreamSynchronize(stream_));
-    const double err = e[1] != 0.0 ? e[0] / std::fabs(e[1]) : (e[0] != 0.0 ? INFINITY : 0.0);
+    auto ratio = [](double num, double den) { return den != 0.0 ? num / std::fabs(den) : (num != 0.0 ? INFINITY : 0.0); };
+    const double err = ratio(e[0], e[1]);
+    double worst = 0.0;
+    uint64_t over = 0;
+    for (size_t s = 0; s < ns; ++s) {
+      const double es = ratio(e[2 + 2 * s], e[3 + 2 * s]);
+      if (!(es <= worst)) worst = es;  // NaN propagates
+      if (!(es <= (double)S.maxErr)) ++over;
+    }
     ++S.rep.check_steps;
     S.rep.last_density_error = err;
     S.rep.max_density_error = std::max(S.rep.max_density_error, err);
+    S.rep.last_max_scene_error = worst;
+    S.rep.last_scenes_over_limit = over;
     S.errSum += err;
-    if (!(err <= (double)S.maxErr)) {
+    if (!(err <= (double)S.maxErr) || over > 0) {
       NBS_HIP(hipMemcpyAsync(buf_.x0, S.shadowX.p, n_ * sizeof(float2), hipMemcpyDeviceToDevice, stream_));
       NBS_HIP(hipMemcpyAsync(buf_.v, S.shadowV.p, n_ * sizeof(float2), hipMemcpyDeviceToDevice, stream_));
       ++S.rep.fallbacks;

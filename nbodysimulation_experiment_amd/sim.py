@@ -285,9 +285,13 @@ class ParticleSimulation:
                 "tile_cap": [int(c) for c in t.tile_cap], "wide_max_particles": int(t.wide_max_particles)}
 
     def set_tuning(self, autotune: bool = True, retune_interval: int = 256, tile_cap=(2048, 2048, 2048),
-                   wide_max_particles: int = 0):
-        t = NbsTuning(1 if autotune else 0, int(retune_interval), (ctypes.c_int32 * 3)(*tile_cap),
-                      int(wide_max_particles))
+                   wide_max_particles: int = 0, tuned: bool = False):
+        """autotune: time the kernel variants on real steps (first all of them,
+        then every retune_interval steps the current choice and its
+        neighbours); tuned=True takes tile_cap as an earlier tuning's result,
+        so only the periodic re-tuning runs."""
+        mode = (2 if tuned else 1) if autotune else 0
+        t = NbsTuning(mode, int(retune_interval), (ctypes.c_int32 * 3)(*tile_cap), int(wide_max_particles))
         call("nbs_set_tuning", self._h, ctypes.byref(t))
 
     def kernel_times(self) -> dict:

@@ -144,6 +144,32 @@ def forward(params, frame, h, dt, bf16_ops=True):
     return o, {"pi": pi, "pj": pj, "F": Fb, "Z": Z, "Hin": Hin, "Min": Min, "G": G, "cnt": cnt, "bf16": bf16_ops}
 
 
+def forward_rows(params, frame, h, dt, rows, bf16_ops=True):
+    """Network outputs o (len(rows) x 2, float64) of the particles ``rows``
+    only, with their neighbours taken from the whole frame: the forward pass
+    of a slice of a frame too large for the all-pairs matrices."""
+    w = unpack(params)
+    rows = np.asarray(rows, np.int64)
+    n = np.asarray(frame["x"]).shape[0]
+    pi, pj = pairs(frame["x"], frame.get("scene", np.zeros(n, np.int32)), h)
+    local = np.full(n, -1, np.int64)
+    local[rows] = np.arange(rows.size)
+    keep = local[pi] >= 0
+    pi, pj = pi[keep], pj[keep]
+    F = features(frame, pi, pj, h, dt, w["norm"])
+    rnd = bf16 if bf16_ops else (lambda a: np.asarray(a, np.float64))
+    Z = rnd(F).astype(np.float64) @ rnd(w["W1"]).astype(np.float64).T
+    li = local[pi]
+    H = np.zeros((rows.size, HID))
+    np.add.at(H, li, np.maximum(Z, 0.0))
+    cnt = np.bincount(li, minlength=rows.size).astype(np.float64)
+    Hin = rnd(H.astype(np.float32)).astype(np.float64) if bf16_ops else H
+    M = Hin @ rnd(w["W2"]).astype(np.float64).T + cnt[:, None] * w["b2"].astype(np.float64)
+    Min = rnd(M.astype(np.float32)).astype(np.float64) if bf16_ops else M
+    G = np.maximum(Min @ rnd(w["W3"]).astype(np.float64).T + w["b3"].astype(np.float64), 0.0)
+    return G @ w["W4"].astype(np.float64).T + w["b4"].astype(np.float64)
+
+
 def loss_and_grad(params, frame, h, dt, bf16_ops=True):
     """Loss and the gradient of the NUM_WEIGHTS trained parameters (float64).
     The backward pass uses the float32 master weights, as the GPU's does."""

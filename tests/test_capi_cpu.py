@@ -58,4 +58,4 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.NbsStats) == 4 * 8 + 9 * 4 + 4  # padded to 8
     assert ctypes.sizeof(_lib.NbsDiagnostics) == 8 + 16 + 8 * 5 + 16 + 8
     assert ctypes.sizeof(_lib.NbsTuning) == 24
-    assert ctypes.sizeof(_lib.NbsSurrogateReport) == 5 * 8 + 3 * 8 + 8
+    assert ctypes.sizeof(_lib.NbsSurrogateReport) == 5 * 8 + 3 * 8 + 8 + 8 + 8

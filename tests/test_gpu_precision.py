@@ -47,6 +47,8 @@ def test_dam_break_1m_fp32_error_against_fp64():
     rows = _curve(dam_break_1m(), [1, 5, 10, 20])
     json.dump(rows, open(os.path.join(OUT, "precision_1m.json"), "w"), indent=1)
     by = {r["step"]: r for r in rows}
-    # positions up to 300h carry ulp(300) = 3.1e-5 h; the error stays within a few ulps early on
-    assert by[10]["max_h"] <= 1e-3, rows
-    assert by[20]["max_h"] <= 3e-3, rows
+    # positions up to 300h carry ulp(300) = 3.1e-5 h.  Measured: 8.0e-5 h at
+    # step 10 and 3.1e-4 h at step 20 (growing as t^2, like the rounding of
+    # the velocity integrated twice); the bounds are twice those.
+    assert by[10]["max_h"] <= 1.6e-4, rows
+    assert by[20]["max_h"] <= 6.4e-4, rows

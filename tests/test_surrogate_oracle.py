@@ -139,3 +139,18 @@ def test_bf16_rounding_ties_to_even():
     got = so.bf16(vals)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), (got, want)
     assert np.isnan(so.bf16(np.array([np.nan], np.float32)))[0]
+
+
+def test_forward_rows_equals_the_full_forward_on_those_rows():
+    """The slice forward (used on frames too large for the all-pairs matrices)
+    gives the full forward's outputs for the chosen rows, bit for bit in
+    float64, with and without the BF16 operand rounding; two scenes, so pairs
+    must not cross scenes."""
+    frame = random_frame(n=120, seed=3)
+    frame["scene"] = (np.arange(120) >= 70).astype(np.int32)
+    p = random_params(seed=4, dtype=np.float32)
+    rows = np.array([0, 5, 17, 69, 70, 71, 118, 119])
+    for bf in (True, False):
+        full, _ = so.forward(p, frame, H, DT, bf16_ops=bf)
+        part = so.forward_rows(p, frame, H, DT, rows, bf16_ops=bf)
+        np.testing.assert_allclose(part, full[rows], rtol=1e-12, atol=1e-14)
