@@ -3,35 +3,59 @@
 Default workload: 1,048,576 particles (1024x1024 block at 0.5h, seeded
 jitter) in a 600h x 600h box with 4 walls and 64 circles; h=1, rho0=10,
 k=0.004, k_near=0.01, sigma=0, beta=0.3, g=(0,-9.8), dt=0.005.  A step is one
-full SPH step of that scene.  W warm-up steps run untimed; then K steps are
-timed with the inputs resident in HBM: the warm-up runs on a throwaway
-instance and the timed steps are steps 0..K-1 of a fresh instance.
+full SPH step of that scene.
 
-roofline: the dominant kernel's algorithmic bytes per launch divided by its
-average duration, measured with HIP events on the library's stream over the
-timed steps.  cpu_baseline: the reference's multi-threaded step restated in
-C++ (oracle/, Gauss-Seidel order, pooled phases), timed on this host on a
-bounded sample that starts from the GPU state at the end of the timed window.
+Protocol (inputs resident in HBM in every timed region):
+  warm-up   W untimed steps on a throwaway instance, continued until the
+            library's autotuner has finished its first tuning; its choice of
+            kernel variants is carried into the timed instances;
+  value     the median over --repeats fresh instances of K timed steps from
+            the scene's initial state (steps 0..K-1), library defaults
+            otherwise (eager launches, no per-kernel events, periodic
+            re-tuning every 256 steps);
+  tuning    the same K steps of a fresh instance that tunes from scratch
+            inside the timed window, reported separately with its overhead.
+
+roofline: the dominant kernel group's limiter.  Its wave64 VALU instructions
+per launch (the rocprofv3 SQ_INSTS_VALU summary of this command in
+profiles/valu_issue.json) over its average launch duration, measured here with
+HIP events on the library's stream, against the VALU issue ceiling of 256 CUs
+x 2.4 GHz x one instruction per CU per cycle; the HBM figure (algorithmic
+bytes per launch over the same duration, against 8 TB/s) follows as the
+secondary bound.
+
+cpu_baseline: the reference's multi-threaded step restated in C++ (oracle/,
+Gauss-Seidel order, pooled phases) on every CPU this process may use, and on
+one thread, timed from GPU states spread over the timed window (steps 0..K);
+the median over the samples.
 
 surrogate (1M dam break): the learned relaxation is trained on the GPU on
 plain frames of 200 seeded 16,384-particle dam breaks (one batched
-simulation, --train-steps Adam steps after --train-warm plain steps), then
-the same K steps are timed with the network relaxing every step and the
-plain density check every 8th step (fallback above 1% density error).
+simulation), then the same K steps are timed with the network relaxing every
+step and the plain density check every 8th step (fallback above 1% density
+error in any scene); beside it the ceiling of any relaxation-only network
+and the trajectory of a 500-step surrogate run against the plain run.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
+import statistics
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s @1M particles: plain vs surrogate at ≤1% density error"
 HBM_PEAK_GBS = 8000.0
+# MI355X: 256 CUs at 2.4 GHz; a CU issues at most one wave64 VALU instruction
+# per cycle (4 SIMD16 units, 4 cycles each).
+VALU_PEAK_GINST = 256 * 2.4
 DT = 0.005
 
 # Algorithmic bytes per particle of each kernel group: every particle record
@@ -45,6 +69,7 @@ KERNEL_BYTES = {"viscosity_predict": 36, "grid": 64, "density": 28, "delta": 44}
 STEP_ALG_BYTES = 172  # reference phases, one SoA FP32 pass each
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
 HID, FEAT = 64, 16
+TRAJECTORY_MARKS = (100, 250, 500)
 
 
 def log(msg):
@@ -67,23 +92,54 @@ def build_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(state, specs, threads, sample_steps, warm_steps=1):
-    """Times the oracle's multi-threaded reference-order step on a sample that
-    starts from the GPU state after warm-up."""
+def host_cpu() -> dict:
+    """The host CPUs this process may use: the affinity mask, capped by the
+    cgroup's CPU quota when there is one (the reference sizes its pool to the
+    logical core count, threading.h:135-138), and the model name (the
+    reference shows it, app.cpp:28-34)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            q, period = fh.read().split()[:2]
+        if q != "max":
+            quota = max(1, math.floor(int(q) / int(period)))
+    except (OSError, ValueError):
+        pass
+    threads = min(affinity, quota) if quota else affinity
+    return {"model": model, "os_cpu_count": os.cpu_count(), "affinity_cpus": affinity, "cgroup_cpu_quota": quota,
+            "threads": threads}
+
+
+def cpu_rates(states, specs, threads, steps):
+    """Particle-steps/s of the oracle's reference-order step (GS_MT: the
+    reference's pooled phases and partitioning) from each state: one untimed
+    step rebuilds the neighbour lists, then `steps` timed steps."""
     from nbodysimulation_experiment_amd.scenes import SPHParams
     from oracle.oracle import GS_MT, OracleSim
-    o = OracleSim(SPHParams(), 32, GS_MT, threads)
-    for s in specs:
-        o.add_scene_spec(s)
-    o.set_gravity(0.0, -9.8)
-    o.set_state(state["x"], state["v"])
-    o.step(DT, warm_steps)  # rebuilds the neighbour lists
-    t0 = time.perf_counter()
-    o.step(DT, sample_steps)
-    t = time.perf_counter() - t0
-    n = o.n
-    o.close()
-    return n * sample_steps / t, t
+    rates = []
+    for st in states:
+        o = OracleSim(SPHParams(), 32, GS_MT, threads)
+        for s in specs:
+            o.add_scene_spec(s)
+        o.set_gravity(0.0, -9.8)
+        o.set_state(st["x"], st["v"])
+        o.step(DT, 1)
+        t0 = time.perf_counter()
+        o.step(DT, steps)
+        t = time.perf_counter() - t0
+        rates.append(o.n * steps / t)
+        o.close()
+    return rates
 
 
 def train_surrogate(args):
@@ -117,12 +173,11 @@ def scenario_benchmark(args):
     iterations per scenario, per-phase min / max / avg) on the GPU, beside the
     reference-order multi-threaded CPU step on the same frames (fewer
     iterations, since its cost is the same in every iteration)."""
-    import numpy as np
-
     from nbodysimulation_experiment_amd import scenarios as sc
     from nbodysimulation_experiment_amd.scenes import volume_positions
     from oracle.oracle import GS_MT, OracleSim
-    threads = args.cpu_threads or min(16, int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1)
+    cpu = host_cpu()
+    threads = args.cpu_threads or cpu["threads"]
     dt = float(sc.CONSTANTS["delta_time"])
     rows = []
     for idx, s in enumerate(sc.SCENARIOS):
@@ -164,10 +219,64 @@ def scenario_benchmark(args):
         log(f"{s.name}: GPU avg {row['gpu_ms'][2]:.3f} ms/frame, CPU ({threads} threads) avg {row['cpu_ms'][2]:.3f}")
     out = {"metric": "built-in benchmark: ms per frame (min, max, avg) per scenario", "unit": "ms",
            "higher_is_better": False, "frames": sc.BENCHMARK_FRAMES, "iterations": sc.BENCHMARK_ITERATIONS,
-           "cpu": {"kind": "port", "cores": threads, "iterations": args.scenario_cpu_iterations,
+           "cpu": {"kind": "port", "cores": threads, "model": cpu["model"], "iterations": args.scenario_cpu_iterations,
                    "order": "reference (Gauss-Seidel, pooled phases)"},
            "dt": dt, "scenarios": rows}
     print(json.dumps(out), flush=True)
+
+
+def first_tuning_steps(n):
+    """Steps until the autotuner's first tuning is done: one step that builds
+    the first binning, then two rounds over the kernel variants (five below
+    131,072 particles, where the wide kernels are a candidate, else four)."""
+    return 1 + 2 * (5 if n <= 131072 else 4)
+
+
+def timed_update(sim, steps):
+    t0 = time.perf_counter()
+    sim.Update(DT, steps)  # returns after the library's stream has drained
+    return time.perf_counter() - t0
+
+
+def compare_states(a, b):
+    """Trajectory divergence of state a from state b (particle-index order)."""
+    xa, xb = a["x"].astype(np.float64), b["x"].astype(np.float64)
+    ra, rb = a["density"][:, 0].astype(np.float64), b["density"][:, 0].astype(np.float64)
+    d = np.linalg.norm(xa - xb, axis=1)
+    ka = 0.5 * float((a["v"].astype(np.float64) ** 2).sum())
+    kb = 0.5 * float((b["v"].astype(np.float64) ** 2).sum())
+    return {"density_error": float(np.abs(ra - rb).sum() / np.abs(rb).sum()),
+            "position_max_h": float(d.max()), "position_mean_h": float(d.mean()),
+            "mean_density_rel": float(abs(ra.mean() - rb.mean()) / rb.mean()),
+            "kinetic_energy_rel": float(abs(ka - kb) / kb) if kb else None}
+
+
+def trajectory(make_sim, spec, params, args):
+    """A surrogate run (checks every --check-interval steps) and a plain run
+    from the same initial state, compared at TRAJECTORY_MARKS; beside them a
+    plain run whose initial positions differ by one ulp, which measures how
+    far the flow itself amplifies the smallest perturbation."""
+    from nbodysimulation_experiment_amd.scenes import SceneSpec
+    bumped = SceneSpec(spec.width, spec.height, spec.cell_size, np.nextafter(spec.positions, np.float32(np.inf)),
+                       planes=spec.planes, circles=spec.circles)
+    plain, ulp, sur = make_sim(), make_sim([bumped]), make_sim()
+    sur.surrogate_set_params(params)
+    sur.surrogate_set_mode(True, args.check_interval, args.max_density_error)
+    out = {"surrogate_vs_plain": {}, "plain_one_ulp_vs_plain": {}}
+    done = 0
+    for m in TRAJECTORY_MARKS:
+        for s in (plain, ulp, sur):
+            s.Update(DT, m - done)
+        done = m
+        p = plain.get_state()
+        out["surrogate_vs_plain"][m] = compare_states(sur.get_state(), p)
+        out["plain_one_ulp_vs_plain"][m] = compare_states(ulp.get_state(), p)
+    rep = sur.surrogate_report()
+    out["surrogate_checks"] = {k: rep[k] for k in ("check_steps", "fallbacks", "max_density_error",
+                                                   "mean_density_error")}
+    for s in (plain, ulp, sur):
+        s.close()
+    return out
 
 
 def main():
@@ -175,16 +284,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--repeats", type=int, default=5, help="timed repeats; value is their median")
     ap.add_argument("--workload", default="dam-break-1m",
                     choices=["dam-break-4k", "dam-break-1m", "batched-256", "batched-surrogate", "scenarios"])
     ap.add_argument("--scenario-cpu-iterations", type=int, default=2)
     ap.add_argument("--graph", action="store_true", help="replay steps as HIP graphs (no per-kernel timing)")
-    ap.add_argument("--cpu-steps", type=int, default=40)
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="timed CPU steps per sample (multi-threaded)")
+    ap.add_argument("--cpu-samples", type=int, default=5, help="CPU samples spread over steps 0..K")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: every CPU this process may use")
+    ap.add_argument("--no-cpu-single-thread", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="also run the library's copy kernel on MiB of data (3 times) for counter calibration")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-relaxation leg")
+    ap.add_argument("--no-trajectory", action="store_true", help="skip the surrogate trajectory comparison")
     ap.add_argument("--no-graph-leg", action="store_true", help="skip the HIP-graph replay leg")
     ap.add_argument("--train-scenes", type=int, default=200)
     ap.add_argument("--train-warm", type=int, default=100)
@@ -205,10 +318,9 @@ def main():
         scenario_benchmark(args)
         return
 
-    from nbodysimulation_experiment_amd.sim import ParticleSimulation
     from nbodysimulation_experiment_amd.scenes import SPHParams
+    from nbodysimulation_experiment_amd.sim import ParticleSimulation, bench_copy
 
-    from nbodysimulation_experiment_amd.sim import bench_copy
     if args.calibrate_copy:
         log(f"copy kernel on {args.calibrate_copy} MiB: {bench_copy(args.calibrate_copy << 20, 3):.1f} GB/s")
     copy_gbs = bench_copy(1 << 30, 20)
@@ -216,98 +328,113 @@ def main():
 
     specs, desc = build_workload(args.workload)
     n = sum(s.n for s in specs)
+    K = args.steps
     log(f"workload {args.workload}: {desc}")
+    fixed_caps = None if args.tile_cap == "auto" else [int(c) for c in args.tile_cap.split(",")]
 
-    def make_sim():
-        sim = ParticleSimulation.from_scene(specs[0], SPHParams()) if len(specs) == 1 else \
-            ParticleSimulation.from_scenes(specs, SPHParams())
-        sim.set_options(hip_graph=args.graph)
-        if args.tile_cap != "auto":
-            sim.set_tuning(autotune=False, tile_cap=[int(c) for c in args.tile_cap.split(",")])
+    def make_sim(scene_specs=None, caps=None, hip_graph=False):
+        """A fresh instance; caps: an earlier tuning's choice (the periodic
+        re-tuning still runs), fixed_caps: no tuning at all."""
+        sp = scene_specs or specs
+        sim = ParticleSimulation.from_scene(sp[0], SPHParams()) if len(sp) == 1 else \
+            ParticleSimulation.from_scenes(sp, SPHParams())
+        sim.set_options(hip_graph=hip_graph or args.graph)
+        if fixed_caps is not None:
+            sim.set_tuning(autotune=False, tile_cap=fixed_caps)
+        elif caps is not None:
+            sim.set_tuning(autotune=True, tile_cap=caps, tuned=True)
         return sim
 
-    # Warm-up on a throwaway instance, then K timed steps of a fresh instance
-    # from the scene's initial state (the inputs are already in HBM).
-    log(f"warm-up {args.warmup} steps")
+    # Warm-up on a throwaway instance, continued until the first tuning is done.
+    warm_steps = args.warmup if fixed_caps is not None else max(args.warmup, first_tuning_steps(n))
+    log(f"warm-up {warm_steps} steps")
     warm = make_sim()
-    warm.Update(DT, args.warmup)
+    warm.Update(DT, warm_steps)
+    caps = list(warm.get_tuning()["tile_cap"]) if fixed_caps is None else fixed_caps
     warm.close()
-    # The headline: the library as a caller runs it by default (per-kernel
-    # event timing off).
-    sim = make_sim()
-    sim.ResetStats()
-    log(f"timing {args.steps} steps")
-    t0 = time.perf_counter()
-    sim.Update(DT, args.steps)  # returns after the stream has drained
-    elapsed = time.perf_counter() - t0
-    value = n * args.steps / elapsed
-    tuning = sim.get_tuning()
-    stats = sim.GetStats()
-    diag = sim.diagnostics()
-    end_state = sim.get_state() if not args.no_cpu_baseline else None
-    sim.close()
-    log(f"{value:.4g} particle-steps/s, {1e3 * elapsed / args.steps:.4f} ms/step")
+    log(f"kernel variants carried into the timed instances: {caps}")
 
-    # Per-kernel averages for the roofline: the same K steps of a fresh
-    # instance with a HIP event pair around every kernel group on the library
-    # stream (the events add a little time at each boundary).
+    # The headline: K steps of fresh instances, median over the repeats.
+    times = []
+    for r in range(args.repeats):
+        sim = make_sim(caps=caps)
+        sim.ResetStats()
+        times.append(timed_update(sim, K))
+        if r == args.repeats - 1:
+            tuning_state = sim.get_tuning()
+            stats = sim.GetStats()
+            diag = sim.diagnostics()
+        sim.close()
+    med = statistics.median(times)
+    value = n * K / med
+    log(f"{value:.4g} particle-steps/s, {1e3 * med / K:.4f} ms/step (median of {len(times)}: "
+        f"{', '.join(f'{1e3 * t / K:.4f}' for t in times)} ms/step)")
+
+    tuning = {"carried_tile_cap": caps, "warm_up_steps": warm_steps, "autotune": fixed_caps is None,
+              "retune_interval": tuning_state["retune_interval"], "tile_cap_at_end": list(tuning_state["tile_cap"])}
+    if fixed_caps is None:
+        # What a fresh caller pays: the first tuning runs inside the window.
+        tsim = make_sim()
+        t_tune = timed_update(tsim, K)
+        tsim.close()
+        tuning["from_scratch_in_window"] = {
+            "value": round(n * K / t_tune, 1), "ms_per_step": round(1e3 * t_tune / K, 5),
+            "tuning_steps": min(K, first_tuning_steps(n)), "overhead_ms": round(1e3 * (t_tune - med), 3)}
+        log(f"tuning from scratch inside the window: {n * K / t_tune:.4g} particle-steps/s")
+
+    # Per-kernel averages for the roofline: the same K steps with a HIP event
+    # pair around every kernel group on the library stream (the events add a
+    # little time at each boundary).
     kt, evented = {}, None
     if not args.graph:
-        esim = make_sim()
+        esim = make_sim(caps=caps)
         esim.set_timing(True)
-        log(f"timing {args.steps} steps with per-kernel events")
-        t0 = time.perf_counter()
-        esim.Update(DT, args.steps)
-        e_elapsed = time.perf_counter() - t0
+        e_el = timed_update(esim, K)
         kt = esim.kernel_times()
         esim.close()
-        evented = {"value": round(n * args.steps / e_elapsed, 1), "unit": "particle-steps/s",
-                   "ms_per_step": round(1e3 * e_elapsed / args.steps, 5)}
-        log(f"with events {n * args.steps / e_elapsed:.4g} particle-steps/s, "
-            f"{1e3 * e_elapsed / args.steps:.4f} ms/step; kernels {kt}")
+        evented = {"value": round(n * K / e_el, 1), "unit": "particle-steps/s", "ms_per_step": round(1e3 * e_el / K, 5),
+                   "kernels_ms": {k: round(v[0], 5) for k, v in kt.items()}}
+        log(f"with events {n * K / e_el:.4g} particle-steps/s; kernels {evented['kernels_ms']}")
 
     graph_leg = None
     if not args.graph and not args.no_graph_leg:
-        # The same K steps of a fresh instance with steady-state steps replayed
-        # as HIP graphs (no per-kernel events; tuning steps still run eagerly).
-        gsim = make_sim()
-        gsim.set_options(hip_graph=True)
-        log(f"timing {args.steps} steps with HIP-graph replay")
-        t0 = time.perf_counter()
-        gsim.Update(DT, args.steps)
-        g_elapsed = time.perf_counter() - t0
+        gsim = make_sim(caps=caps, hip_graph=True)
+        g_el = timed_update(gsim, K)
         gsim.close()
-        graph_leg = {"value": round(n * args.steps / g_elapsed, 1), "unit": "particle-steps/s",
-                     "ms_per_step": round(1e3 * g_elapsed / args.steps, 5)}
-        log(f"graph replay {n * args.steps / g_elapsed:.4g} particle-steps/s")
+        graph_leg = {"value": round(n * K / g_el, 1), "unit": "particle-steps/s", "ms_per_step": round(1e3 * g_el / K, 5)}
+        log(f"graph replay {n * K / g_el:.4g} particle-steps/s")
 
     roofline = None
-    if not args.graph:
+    if kt:
         dom = max(kt, key=lambda k: kt[k][0])
         avg_ms = kt[dom][0]
         alg = KERNEL_BYTES[dom] * n
-        achieved = alg / (avg_ms * 1e-3) / 1e9
+        hbm_gbs = alg / (avg_ms * 1e-3) / 1e9
         traffic = None
         tpath = os.path.join(ROOT, "profiles", "traffic.json")
         if os.path.exists(tpath):
             t = json.load(open(tpath))
             if t.get("workload") == args.workload and dom in t.get("kernels", {}):
                 traffic = t["kernels"][dom]
-        # the kernel's actual limiter: VALU issue, from the instruction-counter profile
+        hbm = {"bound": "hbm", "achieved": round(hbm_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(hbm_gbs / HBM_PEAK_GBS, 5), "traffic": traffic, "alg_bytes_per_launch": alg,
+               "copy_ceiling_gbs": round(copy_gbs, 1),
+               "step_alg_bytes_frac": round(value * STEP_ALG_BYTES / 1e9 / HBM_PEAK_GBS, 5)}
         valu = None
         vpath = os.path.join(ROOT, "profiles", "valu_issue.json")
         if os.path.exists(vpath):
             v = json.load(open(vpath))
-            if v.get("workload") == args.workload and dom in v.get("kernels", {}):
-                valu = v["kernels"][dom]
-        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
-                    "kernel_avg_ms": round(avg_ms, 5), "alg_bytes_per_launch": alg,
-                    "copy_ceiling_gbs": round(copy_gbs, 1),
-                    "step_alg_bytes_frac": round(value * STEP_ALG_BYTES / 1e9 / HBM_PEAK_GBS, 5),
-                    "valu_issue_frac": valu,
-                    "valu_issue_source": "profiles/valu_issue.json (rocprofv3 SQ_INSTS_VALU / GRBM_GUI_ACTIVE)"
-                    if valu is not None else None}
+            ins = v.get("valu_insts_per_launch", {}).get(dom)
+            if v.get("workload") == args.workload and v.get("steps") == K and ins:
+                g = ins / (avg_ms * 1e-3) / 1e9
+                valu = {"bound": "valu", "achieved": round(g, 2), "peak": VALU_PEAK_GINST,
+                        "unit": "G wave64 VALU instructions/s", "frac": round(g / VALU_PEAK_GINST, 5),
+                        "valu_insts_per_launch": ins, "issue_frac_under_counters": v["kernels"].get(dom),
+                        "source": "profiles/valu_issue.json (rocprofv3 SQ_INSTS_VALU of this command)"}
+        primary = valu if valu is not None and valu["frac"] > hbm["frac"] else hbm
+        roofline = dict(primary)
+        roofline.update({"traffic": traffic, "kernel": dom, "kernel_avg_ms": round(avg_ms, 5),
+                         "hbm": hbm, "valu": valu})
 
     surrogate = None
     online = args.workload == "batched-surrogate"  # in-run training on each check step's plain frame
@@ -315,37 +442,44 @@ def main():
         log(f"training the learned relaxation: {args.train_scenes} scenes, {args.train_steps} steps")
         params, train = train_surrogate(args)
         log(f"training: {train}")
-        warm = make_sim()
-        warm.surrogate_set_params(params)
-        warm.surrogate_set_mode(True, args.check_interval, args.max_density_error)
+
+        def sur_sim(timing=False):
+            s = make_sim(caps=caps)
+            s.surrogate_set_params(params)
+            s.surrogate_set_mode(True, args.check_interval, args.max_density_error)
+            if online:
+                s.surrogate_set_online_training(args.lr)
+            s.set_timing(timing)
+            return s
+
+        warm = sur_sim()
         warm.Update(DT, args.warmup)
         warm.close()
-        ssim = make_sim()
-        ssim.surrogate_set_params(params)
-        ssim.surrogate_set_mode(True, args.check_interval, args.max_density_error)
-        if online:
-            ssim.surrogate_set_online_training(args.lr)
-        ssim.set_timing(True)
-        ssim.ResetStats()
-        log(f"timing {args.steps} surrogate steps")
-        t0 = time.perf_counter()
-        ssim.Update(DT, args.steps)
-        s_elapsed = time.perf_counter() - t0
-        s_kt = ssim.kernel_times()
-        rep = ssim.surrogate_report()
-        sdiag = ssim.diagnostics()
-        scene_err = ssim.surrogate_scene_errors() if online else None
-        ssim.close()
-        s_value = n * args.steps / s_elapsed
-        plain_evented = evented["value"] if evented else value
+        s_times = []
+        for _ in range(max(1, min(3, args.repeats))):
+            ssim = sur_sim()
+            s_times.append(timed_update(ssim, K))
+            rep = ssim.surrogate_report()
+            ssim.close()
+        s_med = statistics.median(s_times)
+        s_value = n * K / s_med
+        esim = sur_sim(timing=True)
+        s_el = timed_update(esim, K)
+        s_kt = esim.kernel_times()
+        erep = esim.surrogate_report()
+        sdiag = esim.diagnostics()
+        scene_err = esim.surrogate_scene_errors() if online else None
+        esim.close()
         relax_ms = s_kt["delta"][0]
-        pairs = rep["last_pair_count"]
+        pairs = erep["last_pair_count"]
         flops = 2.0 * HID * FEAT * pairs + 2.0 * 2 * HID * HID * n
+        plain_ms = 1e3 * med / K
         surrogate = {
-            "value": round(s_value, 1), "unit": "particle-steps/s", "ms_per_step": round(1e3 * s_elapsed / args.steps, 5),
-            # both runs with per-kernel events
-            "speedup_vs_plain": round(s_value / plain_evented, 4),
-            "density_error": {"definition": "sum|rho(surrogate) - rho(plain)| / sum rho(plain) on check steps",
+            "value": round(s_value, 1), "unit": "particle-steps/s", "ms_per_step": round(1e3 * s_med / K, 5),
+            "speedup_vs_plain": round(s_value / value, 4),
+            "speedup_vs_plain_with_events": round((n * K / s_el) / evented["value"], 4) if evented else None,
+            "density_error": {"definition": "sum|rho(surrogate) - rho(plain)| / sum rho(plain) on check steps, "
+                                            "from the same input (one step)",
                               "max": rep["max_density_error"], "mean": rep["mean_density_error"],
                               "limit": args.max_density_error, "check_interval": args.check_interval,
                               "checks": rep["check_steps"], "fallbacks": rep["fallbacks"]},
@@ -359,7 +493,7 @@ def main():
                              "phase's average time (check steps add a plain relaxation and two density passes)"},
             "training": train,
             "in_run_training": None if not online else {
-                "learning_rate": args.lr, "adam_steps": rep["train_steps"],
+                "learning_rate": args.lr, "adam_steps": erep["train_steps"],
                 "scene_density_error": {"scenes": int(scene_err.size), "min": float(scene_err.min()),
                                         "max": float(scene_err.max()), "mean": float(scene_err.mean()),
                                         "largest_scene": float(scene_err[-1]),
@@ -367,25 +501,60 @@ def main():
             "kernels_ms": {k: round(v[0], 5) for k, v in s_kt.items()},
             "diagnostics_fp64": {k: sdiag[k] for k in ("mean_density", "max_density", "non_finite_count")},
         }
-        log(f"surrogate {s_value:.4g} particle-steps/s ({s_value / plain_evented:.3f}x plain); report {rep}")
+        if kt:
+            step_ms = sum(v[0] for v in kt.values())
+            surrogate["ceiling_vs_plain"] = {
+                "value": round(step_ms / (step_ms - kt["delta"][0]), 4),
+                "definition": "plain step / (plain step - plain relaxation kernel), per-kernel event averages: the "
+                              "speed-up of a relaxation network that cost nothing (the relaxation kernel also "
+                              "holds collisions and the velocity update, which the network step still runs)"}
+        log(f"surrogate {s_value:.4g} particle-steps/s ({s_value / value:.3f}x plain, plain {plain_ms:.4f} ms/step); "
+            f"report {rep}")
+        if args.workload == "dam-break-1m" and not args.no_trajectory:
+            log(f"trajectory: surrogate and plain runs to step {TRAJECTORY_MARKS[-1]}")
+            surrogate["trajectory"] = trajectory(lambda sp=None: make_sim(sp, caps=caps), specs[0], params, args)
+            log(f"trajectory: {surrogate['trajectory']}")
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1)
-        log(f"cpu baseline: {threads} threads, {args.cpu_steps} steps from step {args.steps}")
-        cv, ct = cpu_baseline(end_state, specs, threads, args.cpu_steps)
+        host = host_cpu()
+        threads = args.cpu_threads or host["threads"]
+        marks = sorted({round(k * K / max(1, args.cpu_samples - 1)) for k in range(args.cpu_samples)})
+        ssim = make_sim(caps=caps)
+        states, done = [], 0
+        for m in marks:
+            ssim.Update(DT, m - done)
+            done = m
+            s = ssim.get_state()
+            states.append({"x": s["x"], "v": s["v"]})
+        ssim.close()
+        log(f"cpu baseline: {threads} threads, {args.cpu_steps} steps from each of steps {marks}")
+        t0 = time.perf_counter()
+        mt = cpu_rates(states, specs, threads, args.cpu_steps)
+        t_mt = time.perf_counter() - t0
+        cv = statistics.median(mt)
         cpu = {"value": round(cv, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the same scene from the GPU state at the end of the timed "
-                         f"window (step {args.steps}; +1 untimed step to rebuild the neighbour lists), {ct:.1f} s",
-               "gpu_over_cpu": round(value / cv, 1)}
-        log(f"cpu baseline {cv:.4g} particle-steps/s")
+               "model": host["model"], "host": host,
+               "sample": f"median of {len(mt)} samples: {args.cpu_steps} timed steps of the same scene from the GPU "
+                         f"states at steps {marks} (the timed window; +1 untimed step each to rebuild the neighbour "
+                         f"lists), {t_mt:.1f} s in all",
+               "samples": [round(r, 1) for r in mt], "gpu_over_cpu": round(value / cv, 1)}
+        log(f"cpu baseline {cv:.4g} particle-steps/s ({threads} threads)")
+        if not args.no_cpu_single_thread:
+            st = cpu_rates(states, specs, 1, 1)
+            cpu["single_thread"] = {"value": round(statistics.median(st), 1), "cores": 1,
+                                    "samples": [round(r, 1) for r in st],
+                                    "sample": f"median of {len(st)} samples, 1 timed step from each state",
+                                    "gpu_over_cpu": round(value / statistics.median(st), 1)}
+            log(f"cpu single thread {statistics.median(st):.4g} particle-steps/s")
 
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(1e3 * elapsed / args.steps, 5), "higher_is_better": True,
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * med / K, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": desc, "particles": n, "dt": DT, "method": "plain", "hip_graph": args.graph,
-                   "tuning": tuning},
+                   "repeats": len(times), "tuning": tuning},
+        "repeats_ms_per_step": [round(1e3 * t / K, 5) for t in times],
         "roofline": roofline, "cpu_baseline": cpu, "per_kernel_events": evented, "hip_graph_replay": graph_leg,
         "surrogate": surrogate,
         "stats": {k: v for k, v in stats.items() if k != "time"},

@@ -2,7 +2,7 @@
 
     python -m nbodysimulation_experiment_amd.profiling stats  <kernel_stats.csv>
     python -m nbodysimulation_experiment_amd.profiling counters <counter_collection.csv> [--copy-bytes N]
-                                                              [--workload W --json valu.json]
+                                                              [--workload W --steps K --json valu.json]
     python -m nbodysimulation_experiment_amd.profiling traffic <fetch.csv> <write.csv> --copy-bytes N
                                                               --workload W [--json out.json]
     python -m nbodysimulation_experiment_amd.profiling steps <counter_collection.csv> [--ranges 2,100,300,400,502]
@@ -120,11 +120,14 @@ def traffic(fetch_csv: str, write_csv: str, copy_bytes: float, workload: str) ->
 CUS = 256  # MI355X compute units; GRBM_GUI_ACTIVE is summed over its 8 XCDs
 
 
-def valu_issue(c: dict, workload: str) -> dict:
+def valu_issue(c: dict, workload: str, steps: int = 0) -> dict:
     """VALU issue utilisation of each kernel group from a counters() result:
     SQ_INSTS_VALU / (CUS * GRBM_GUI_ACTIVE / 8), a CU issuing at most one wave64
     VALU instruction per cycle; the single-scene LDS-tile variants pooled by
-    dispatches (multi-scene variants, "<t,N>", belong to other workloads)."""
+    dispatches (multi-scene variants, "<t,N>", belong to other workloads).
+    Also the wave64 VALU instructions per launch of each group (summed over
+    the group's kernels), which bench.py divides by its measured launch
+    duration for the same command (``steps`` timed steps)."""
     per = collections.defaultdict(lambda: [0.0, 0.0])  # kernel: [instr, cycles] summed over dispatches
     for k, v in c.items():
         if "SQ_INSTS_VALU" not in v or "GRBM_GUI_ACTIVE" not in v or "<t," in k:
@@ -132,15 +135,21 @@ def valu_issue(c: dict, workload: str) -> dict:
         name = k.split("<")[0]
         per[name][0] += v["SQ_INSTS_VALU"] * v["dispatches"]
         per[name][1] += v["GRBM_GUI_ACTIVE"] * v["dispatches"]
-    groups = {}
+    disp = collections.Counter()
+    for k, v in c.items():
+        if "SQ_INSTS_VALU" in v and "<t," not in k:
+            disp[k.split("<")[0]] += v["dispatches"]
+    groups, per_launch = {}, {}
     for g, variants in GROUPS.items():
         for members in variants:  # the first alternative that ran, its kernels together
             if all(k in per for k in members):
                 ins = sum(per[k][0] for k in members)
                 cyc = sum(per[k][1] for k in members)
                 groups[g] = round(ins / (CUS * cyc / 8.0), 4)
+                per_launch[g] = round(sum(per[k][0] / disp[k] for k in members))
                 break
-    return {"workload": workload, "unit": "fraction of VALU issue slots", "kernels": groups}
+    return {"workload": workload, "steps": steps, "unit": "fraction of VALU issue slots", "kernels": groups,
+            "valu_insts_per_launch": per_launch}
 
 
 def step_ranges(path: str, bounds: list[int]) -> list[dict]:
@@ -189,6 +198,7 @@ def main(argv=None):
     ap.add_argument("--workload", default="dam-break-1m")
     ap.add_argument("--json", default="")
     ap.add_argument("--ranges", default="2,100,300,400,502")
+    ap.add_argument("--steps", type=int, default=0, help="timed steps of the profiled bench.py command")
     a = ap.parse_args(argv)
     if a.mode == "steps":
         for row in step_ranges(a.path, [int(x) for x in a.ranges.split(",")]):
@@ -220,7 +230,7 @@ def main(argv=None):
     c = counters(a.path)
     if a.json:
         with open(a.json, "w") as fh:
-            fh.write(json.dumps(valu_issue(c, a.workload), indent=1) + "\n")
+            fh.write(json.dumps(valu_issue(c, a.workload, a.steps), indent=1) + "\n")
     if a.copy_bytes and "copy" in c:
         for key in ("FETCH_SIZE", "WRITE_SIZE"):
             if key in c["copy"]:

@@ -30,3 +30,27 @@ def test_more_than_one_gpu_is_refused():
                        text=True, timeout=120, cwd=ROOT)
     assert r.returncode != 0
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_host_cpu_reports_usable_threads_and_model():
+    h = bench.host_cpu()
+    assert 1 <= h["threads"] <= h["affinity_cpus"] <= (os.cpu_count() or h["affinity_cpus"])
+    assert h["threads"] == len(os.sched_getaffinity(0)) or h["cgroup_cpu_quota"] == h["threads"]
+    assert h["model"]
+
+
+def test_first_tuning_steps_cover_two_rounds_of_every_variant():
+    # one binning step, then 2 rounds over 0/1024/2048/4096 (+ the wide kernels up to 131,072 particles)
+    assert bench.first_tuning_steps(1 << 20) == 1 + 2 * 4
+    assert bench.first_tuning_steps(4096) == 1 + 2 * 5
+
+
+def test_compare_states():
+    import numpy as np
+    rng = np.random.default_rng(0)
+    a = {"x": rng.normal(size=(50, 2)).astype(np.float32), "v": rng.normal(size=(50, 2)).astype(np.float32),
+         "density": rng.uniform(1, 3, (50, 2)).astype(np.float32)}
+    z = bench.compare_states(a, a)
+    assert all(v == 0 for v in z.values())
+    b = dict(a, x=a["x"] + np.float32(0.5))
+    assert abs(bench.compare_states(b, a)["position_max_h"] - 0.5 * 2 ** 0.5) < 1e-5
