@@ -225,6 +225,24 @@ nbs_status nbs_get_tuning(nbs_sim *sim, nbs_tuning *tuning);
 /* FP64 diagnostics over the current state. */
 nbs_status nbs_get_diagnostics(nbs_sim *sim, nbs_diagnostics *out);
 
+/* The diagnostics of nbs_get_diagnostics over caller-owned device arrays,
+ * allocating nothing.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).
+ *   x, v, density: device pointers (8-byte aligned) to n interleaved float
+ *     pairs each: positions, velocities, (density, nearDensity);
+ *   out: a device pointer to one nbs_diagnostics, written by the call's last
+ *     kernel; the count and mean fields are over the n particles;
+ *   stream: a hipStream_t to enqueue on (NULL: the null stream); the call
+ *     returns after enqueueing, without synchronising.
+ * The partial sums use a fixed 384 KiB device array per GPU, held until the
+ * process ends, so n is at most NBS_DIAGNOSTICS_NOALLOC_MAX_PARTICLES
+ * (NBS_ERR_CAPACITY beyond).  Calls that overlap on one GPU share that array
+ * and race: serialise them (one stream, or events). */
+#define NBS_DIAGNOSTICS_NOALLOC_MAX_PARTICLES 16777216u
+nbs_status nbs_diagnostics_noalloc(const float *x, const float *v, const float *density, uint64_t n,
+                                   float rest_density, nbs_diagnostics *out, void *stream);
+
 /* The cell grid of the last binning: sorted_ids[n] lists particle indices in
  * cell order (cell key, then particle index); cell_start[total_cells + 1]
  * holds each cell's first position in sorted_ids. */

@@ -117,6 +117,7 @@ SIGNATURES = {
     "nbs_set_timing": [vp, c_int32],
     "nbs_set_options": [vp, c_uint32],
     "nbs_get_diagnostics": [vp, P(NbsDiagnostics)],
+    "nbs_diagnostics_noalloc": [vp, vp, vp, c_uint64, c_float, vp, vp],
     "nbs_set_tuning": [vp, P(NbsTuning)],
     "nbs_get_tuning": [vp, P(NbsTuning)],
     "nbs_get_total_cells": [vp, P(c_uint64)],

@@ -934,8 +934,8 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *p, uint32_t value
 // FP64 diagnostics: per-workgroup partial sums over fixed slot ranges, then one
 // workgroup combines them in workgroup order.
 // ---------------------------------------------------------------------------
-constexpr int kDiag = 12;  // vx, vy, |v|^2, rho, max rho, |rho-rho0|, (rho-rho0)^2, x, y, non-finite, count, pad
-constexpr int kDiagItems = 16;
+constexpr int kDiag = kDiagWidth;
+constexpr int kDiagItems = kDiagPerBlock / kBlock;
 
 __device__ __forceinline__ double block_sum_d(double v, double *sh) {
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
@@ -1005,6 +1005,24 @@ __global__ __launch_bounds__(kBlock) void k_diag_final(int nblocks, const double
     }
   }
   for (int k = 0; k < kDiag; ++k) out[k] = r[k];
+}
+
+// Combines the partials like k_diag_final and writes the record.
+__global__ __launch_bounds__(kBlock) void k_diag_record(int nblocks, const double *partials, uint64_t n, double rho0,
+                                                        nbs_diagnostics *out) {
+  if (threadIdx.x != 0) return;
+  double r[kDiag];
+  for (int k = 0; k < kDiag; ++k) r[k] = 0.0;
+  r[4] = -1.0e300;
+  for (int bI = 0; bI < nblocks; ++bI) {
+    for (int k = 0; k < kDiag; ++k) {
+      const double v = partials[(size_t)bI * kDiag + k];
+      r[k] = (k == 4) ? fmax(r[k], v) : r[k] + v;
+    }
+  }
+  nbs_diagnostics d;
+  diagnostics_record(r, n, rho0, &d);
+  *out = d;
 }
 
 int diagnostics_blocks(int n) { return (n + kBlock * kDiagItems - 1) / (kBlock * kDiagItems); }
@@ -1176,6 +1194,20 @@ void launch_diagnostics(const Buffers &b, const DevParams &p, int n, double *par
   const int nb = diagnostics_blocks(n);
   if (nb > 0) hipLaunchKernelGGL(k_diag_partial, dim3(nb), dim3(kBlock), 0, s, b, p, n, partials);
   hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(kBlock), 0, s, nb, partials, out);
+}
+
+void launch_diagnostics_record(const float2 *x, const float2 *v, const float2 *dens, int n, float rho0,
+                               double *partials, nbs_diagnostics *out, hipStream_t s) {
+  Buffers b{};
+  b.x0 = const_cast<float2 *>(x);
+  b.v = const_cast<float2 *>(v);
+  b.dens = const_cast<float2 *>(dens);
+  DevParams p{};
+  p.rho0 = rho0;
+  const int nb = diagnostics_blocks(n);
+  if (nb > 0) hipLaunchKernelGGL(k_diag_partial, dim3(nb), dim3(kBlock), 0, s, b, p, n, partials);
+  hipLaunchKernelGGL(k_diag_record, dim3(1), dim3(kBlock), 0, s, nb, (const double *)partials, (uint64_t)n,
+                     (double)rho0, out);
 }
 
 void launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t s) {

@@ -2,7 +2,10 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
+
+#include "../../include/nbs_sph.h"
 
 namespace nbs {
 
@@ -120,9 +123,36 @@ void launch_gather_state(const Buffers &b, const DevScene &s0, int n, int nBinne
                          float2 *dens, float2 *pres, uint32_t *key, int32_t *scene, hipStream_t stream);
 void launch_colors(const Buffers &b, float rho0, int n, float4 *rgba, hipStream_t stream);
 void launch_fill_u32(uint32_t *p, uint32_t value, uint64_t n, hipStream_t stream);
+// FP64 diagnostics.  The sums (kDiagWidth doubles: sum vx, sum vy, sum |v|^2,
+// sum rho, max rho, sum |rho - rho0|, sum (rho - rho0)^2, sum x, sum y,
+// non-finite count, finite count, 0) are reduced per workgroup of
+// kDiagPerBlock slots into partials[diagnostics_blocks(n) * kDiagWidth], then
+// combined in workgroup order.
+constexpr int kDiagWidth = 12;
+constexpr int kDiagPerBlock = kBlock * 16;
 void launch_diagnostics(const Buffers &b, const DevParams &p, int n, double *partials, double *out,
                         hipStream_t stream);
 int diagnostics_blocks(int n);
+// The same reduction over caller-owned SoA arrays, finished on the device into
+// one nbs_diagnostics record at `out` (device memory); n particles.
+void launch_diagnostics_record(const float2 *x, const float2 *v, const float2 *dens, int n, float rho0,
+                               double *partials, nbs_diagnostics *out, hipStream_t stream);
+
+// The nbs_diagnostics record of n particles from the combined sums r.
+__host__ __device__ inline void diagnostics_record(const double *r, uint64_t n, double rho0, nbs_diagnostics *d) {
+  const double cnt = r[10] > 0 ? r[10] : 1.0;
+  d->particle_count = n;
+  d->momentum[0] = r[0];
+  d->momentum[1] = r[1];
+  d->kinetic_energy = 0.5 * r[2];
+  d->mean_density = r[3] / cnt;
+  d->max_density = n ? r[4] : 0.0;
+  d->mean_abs_density_error = rho0 != 0.0 ? (r[5] / cnt) / fabs(rho0) : 0.0;
+  d->rms_density_error = rho0 != 0.0 ? sqrt(r[6] / cnt) / fabs(rho0) : 0.0;
+  d->centroid[0] = r[7] / cnt;
+  d->centroid[1] = r[8] / cnt;
+  d->non_finite_count = (uint64_t)r[9];
+}
 void launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t stream);
 void launch_selftest_math(const float *a, const float *b, uint64_t n, float *out, hipStream_t stream);
 

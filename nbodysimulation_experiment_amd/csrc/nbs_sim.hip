@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -355,25 +356,14 @@ class Simulation {
     d.particle_count = n_;
     if (n_ == 0) return d;
     const int nb = diagnostics_blocks((int)n_);
-    if (diagPartials_.n < (size_t)nb * 12) diagPartials_.alloc((size_t)nb * 12);
-    if (diagOut_.n < 12) diagOut_.alloc(12);
+    if (diagPartials_.n < (size_t)nb * kDiagWidth) diagPartials_.alloc((size_t)nb * kDiagWidth);
+    if (diagOut_.n < kDiagWidth) diagOut_.alloc(kDiagWidth);
     launch_diagnostics(buf_, devParams(0.0f), (int)n_, diagPartials_.p, diagOut_.p, stream_);
     NBS_HIP(hipGetLastError());
-    double r[12];
+    double r[kDiagWidth];
     NBS_HIP(hipMemcpyAsync(r, diagOut_.p, sizeof(r), hipMemcpyDeviceToHost, stream_));
     NBS_HIP(hipStreamSynchronize(stream_));
-    const double cnt = r[10] > 0 ? r[10] : 1.0;
-    const double rho0 = params_.rest_density;
-    d.momentum[0] = r[0];
-    d.momentum[1] = r[1];
-    d.kinetic_energy = 0.5 * r[2];
-    d.mean_density = r[3] / cnt;
-    d.max_density = r[4];
-    d.mean_abs_density_error = rho0 != 0.0 ? (r[5] / cnt) / std::fabs(rho0) : 0.0;
-    d.rms_density_error = rho0 != 0.0 ? std::sqrt(r[6] / cnt) / std::fabs(rho0) : 0.0;
-    d.centroid[0] = r[7] / cnt;
-    d.centroid[1] = r[8] / cnt;
-    d.non_finite_count = (uint64_t)r[9];
+    diagnostics_record(r, n_, (double)params_.rest_density, &d);
     return d;
   }
 
@@ -1625,6 +1615,57 @@ nbs_status nbs_get_diagnostics(nbs_sim *sim, nbs_diagnostics *out) {
   NBS_CHECK_SIM(sim);
   if (!out) return NBS_ERR_INVALID_ARGUMENT;
   return guarded([&] { *out = sim->impl->diagnostics(); });
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics without allocation (nbs_diagnostics_noalloc).  The partial sums
+// live in `scratch`, one fixed array per GPU for the life of the process: one
+// row of kDiagWidth doubles per workgroup of kDiagPerBlock particles, for at
+// most NBS_DIAGNOSTICS_NOALLOC_MAX_PARTICLES particles (384 KiB).  Every row
+// that a call reads is written by the same call, so nothing needs clearing;
+// calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+namespace nbs {
+constexpr size_t kNoAllocRows = NBS_DIAGNOSTICS_NOALLOC_MAX_PARTICLES / kDiagPerBlock;
+static_assert(NBS_DIAGNOSTICS_NOALLOC_MAX_PARTICLES % kDiagPerBlock == 0, "whole workgroups");
+static __device__ __align__(256) double scratch[kNoAllocRows * kDiagWidth];
+
+// Device address of `scratch` on each GPU, looked up on the first call with
+// that GPU current.  The host-side name is only a stand-in for the symbol.
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratchMutex;
+static double *g_scratchAddr[kMaxDevices] = {};
+
+static double *scratch_on_current_device() {
+  int dev = 0;
+  NBS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw Error(NBS_ERR_UNSUPPORTED, "device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratchMutex);
+  if (!g_scratchAddr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      throw Error(NBS_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratchAddr[dev] = static_cast<double *>(ptr);
+  }
+  return g_scratchAddr[dev];
+}
+}  // namespace nbs
+
+nbs_status nbs_diagnostics_noalloc(const float *x, const float *v, const float *density, uint64_t n,
+                                   float rest_density, nbs_diagnostics *out, void *stream) {
+  if (!out || (n > 0 && (!x || !v || !density)))
+    return (nbs::g_lastError = "null argument", NBS_ERR_INVALID_ARGUMENT);
+  if (n > NBS_DIAGNOSTICS_NOALLOC_MAX_PARTICLES)
+    return (nbs::g_lastError = "n exceeds NBS_DIAGNOSTICS_NOALLOC_MAX_PARTICLES (16,777,216)", NBS_ERR_CAPACITY);
+  if (!std::isfinite(rest_density)) return (nbs::g_lastError = "rest_density must be finite", NBS_ERR_INVALID_ARGUMENT);
+  return guarded([&] {
+    double *partials = nbs::scratch_on_current_device();
+    nbs::launch_diagnostics_record(reinterpret_cast<const float2 *>(x), reinterpret_cast<const float2 *>(v),
+                                   reinterpret_cast<const float2 *>(density), (int)n, rest_density, partials, out,
+                                   static_cast<hipStream_t>(stream));
+    NBS_HIP(hipGetLastError());
+  });
 }
 
 nbs_status nbs_get_total_cells(nbs_sim *sim, uint64_t *total_cells) {

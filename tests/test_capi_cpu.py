@@ -59,3 +59,16 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.NbsDiagnostics) == 8 + 16 + 8 * 5 + 16 + 8
     assert ctypes.sizeof(_lib.NbsTuning) == 24
     assert ctypes.sizeof(_lib.NbsSurrogateReport) == 5 * 8 + 3 * 8 + 8 + 8 + 8
+
+
+def test_noalloc_diagnostics_argument_checks_without_a_device():
+    """nbs_diagnostics_noalloc rejects null arrays and sizes beyond its fixed
+    scratch array before it touches the GPU."""
+    L = _lib.load()
+    fake = ctypes.c_void_p(0x1000)  # never dereferenced: the checks come first
+    assert L.nbs_diagnostics_noalloc(fake, fake, fake, 10, ctypes.c_float(10.0), None, None) == 1
+    assert L.nbs_diagnostics_noalloc(None, fake, fake, 10, ctypes.c_float(10.0), fake, None) == 1
+    too_many = 16777216 + 1
+    assert L.nbs_diagnostics_noalloc(fake, fake, fake, too_many, ctypes.c_float(10.0), fake, None) == 4
+    assert "16,777,216" in L.nbs_last_error().decode()
+    assert L.nbs_diagnostics_noalloc(fake, fake, fake, 10, ctypes.c_float(float("nan")), fake, None) == 1
