@@ -244,7 +244,12 @@ nbs_status nbs_diagnostics_noalloc(const float *x, const float *v, const float *
                                    float rest_density, nbs_diagnostics *out, void *stream);
 
 /* The cell grid of the last binning: sorted_ids[n] lists particle indices in
- * cell order (cell key, then particle index); cell_start[total_cells + 1]
+ * cell order: by cell key, each cell in the order of its particles' slots
+ * before that binning (a stable counting sort of the slot order; the slot
+ * order starts in particle-index order and restarts there after
+ * nbs_set_state, and particles added since the last binning follow in index
+ * order), so a first binning lists each cell by ascending index, the
+ * reference's order after AddParticle (demo4.cpp:46-47); cell_start[total_cells + 1]
  * holds each cell's first position in sorted_ids. */
 nbs_status nbs_get_total_cells(nbs_sim *sim, uint64_t *total_cells);
 nbs_status nbs_get_grid(nbs_sim *sim, uint32_t *sorted_ids, uint32_t *cell_start);

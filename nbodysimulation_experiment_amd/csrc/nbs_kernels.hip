@@ -7,8 +7,9 @@
 //   2. k_scan_reduce + k_scan_final (k_scan_small for small grids)
 //                       exclusive scan of the per-cell counts -> cell starts
 //   3. k_scatter + k_rank_reorder
-//                       bucket the particles by cell, order each cell by particle
-//                       index, and move the state into the new slot order
+//                       bucket the particles by cell, order each cell by the
+//                       particles' previous slots (a stable counting sort of the
+//                       slot order), and move the state into the new slot order
 //   4. k_density        density, near density, pressure (demo4.cpp:208-221)
 //   5. k_delta          double-density relaxation (Jacobi gather), plane /
 //                       circle / segment / polygon collisions, velocity
@@ -227,6 +228,24 @@ __device__ __forceinline__ uint32_t wave_aggregated_fill(uint32_t *count, uint32
   return base + rank;
 }
 
+// Appends `value` of every lane with `flag` to list[*counter ...] with one
+// atomic per wave; entries at cap or beyond are dropped.  All active lanes must
+// call it.
+__device__ __forceinline__ void wave_append(uint32_t *list, uint32_t *counter, uint32_t cap, bool flag,
+                                            uint32_t value) {
+  const uint64_t m = __ballot(flag);
+  if (m == 0ull) return;  // wave-uniform
+  const int lane = (int)(threadIdx.x & 63u);
+  const int leader = __builtin_ctzll(m);
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = (uint32_t)__shfl((int)base, leader, 64);
+  const uint32_t at = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+  if (flag && at < cap) list[at] = value;
+}
+
+constexpr uint32_t kFarFlag = 0x80000000u;  // bit 31 of keyp in placement mode
+
 // ---------------------------------------------------------------------------
 // 1. integrate + viscosity + predict + key + count
 // ---------------------------------------------------------------------------
@@ -302,15 +321,16 @@ __global__ __launch_bounds__(kBlock) void k_visc_predict(Buffers b, DevParams p,
   float vx = (aX * p.dt) + vi.x;                         // demo4.cpp:307
   float vy = (aY * p.dt) + vi.y;
 
+  int pcx = -2, pcy = -2;  // cell of the last binning (none: not adjacent to any cell)
   if (lists && binned) {
     const uint32_t local = b.keys[i] - sc.cellBase;
-    const int cy = (int)(local / (uint32_t)sc.nx);
-    const int cx = (int)local - cy * sc.nx;
+    pcy = (int)(local / (uint32_t)sc.nx);
+    pcx = (int)local - pcy * sc.nx;
     float accx = 0.0f, accy = 0.0f;
     if (CAP > 0 && plan.use)
-      viscosity_sum(b.startCur, sc, cx, cy, tx, b.v, TileIndex(plan), p, xi, vx, vy, accx, accy);
+      viscosity_sum(b.startCur, sc, pcx, pcy, tx, b.v, TileIndex(plan), p, xi, vx, vy, accx, accy);
     else
-      viscosity_sum(b.startCur, sc, cx, cy, b.x0, b.v, SlotIndex(), p, xi, vx, vy, accx, accy);
+      viscosity_sum(b.startCur, sc, pcx, pcy, b.x0, b.v, SlotIndex(), p, xi, vx, vy, accx, accy);
     vx = accx + vx;
     vy = accy + vy;
   }
@@ -321,8 +341,17 @@ __global__ __launch_bounds__(kBlock) void k_visc_predict(Buffers b, DevParams p,
   const int cx = cell_coord(xpx, sc.halfW, sc.cellSize, sc.nx);
   const int cy = cell_coord(xpy, sc.halfH, sc.cellSize, sc.ny);
   const uint32_t key = sc.cellBase + (uint32_t)(cy * sc.nx + cx);
-  b.keyp[i] = key;
-  b.fill[i] = wave_aggregated_fill(b.count, key);
+  if (b.farList) {
+    // placement mode: counts only (k_place orders the cells); a particle that
+    // left the 3x3 cells around its last cell is flagged and listed
+    const bool far = abs(cx - pcx) > 1 || abs(cy - pcy) > 1;
+    b.keyp[i] = key | (far ? kFarFlag : 0u);
+    (void)wave_aggregated_fill(b.count, key);
+    wave_append(b.farList, b.farCount, (uint32_t)n, far, (uint32_t)i);
+  } else {
+    b.keyp[i] = key;
+    b.fill[i] = wave_aggregated_fill(b.count, key);
+  }
 }
 
 // Restores the per-slot scene and zero acceleration of particles that lose
@@ -410,9 +439,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(Buffers b, uint32_t cell
     b.cellStats[2 * blockIdx.x + 1] = mx;
   }
   if (threadIdx.x == 0) b.blockSum[blockIdx.x] = tot;
+  // the next step's far-mover counter (its previous user, k_place of the last
+  // step, has finished)
+  if (blockIdx.x == 0 && threadIdx.x == 0 && b.farCountClear) *b.farCountClear = 0u;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_final(Buffers b, uint32_t cells) {
+__global__ __launch_bounds__(kBlock) void k_scan_final(Buffers b, uint32_t cells, uint32_t n) {
   const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
   uint32_t s = 0;
@@ -440,6 +472,15 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(Buffers b, uint32_t cells
     if (c < cells) {
       b.startNext[c] = run;
       b.count[c] = 0u;  // ready for the next step's counting
+      if (b.chunkCells && v[k] > 0u) {
+        // placement chunks whose first or last slot lies in this cell
+        const uint32_t first = run, last = run + v[k] - 1u, lastChunk = (n - 1u) / kPlaceChunk;
+        for (uint32_t g = (first + kPlaceChunk - 1u) / kPlaceChunk; g * kPlaceChunk <= last; ++g)
+          b.chunkCells[2 * g] = (uint32_t)c;
+        for (uint32_t g = (first + 1u + kPlaceChunk - 1u) / kPlaceChunk; g <= (last + 1u) / kPlaceChunk; ++g)
+          if (g >= 1u && g - 1u < lastChunk) b.chunkCells[2 * (g - 1u) + 1] = (uint32_t)c;
+        if (last == n - 1u) b.chunkCells[2 * lastChunk + 1] = (uint32_t)c;
+      }
       run += v[k];
       if (c == (uint64_t)cells - 1) b.startNext[cells] = run;
     }
@@ -455,35 +496,36 @@ __global__ __launch_bounds__(kBlock) void k_scatter(Buffers b, int n) {
   const uint32_t key = b.keyp[s];
   const uint32_t pos = b.startNext[key] + b.fill[s];
   b.tmpSlot[pos] = (uint32_t)s;
-  b.tmpId[pos] = b.id[s];
 }
 
 // A workgroup's slots are consecutive in cell order, so the cells they belong
 // to span [c0 of its first slot, c1 of its last slot); when that span fits
-// kRankTile, its particle indices are staged in LDS for the ranking loops.
+// kRankTile, its source slots are staged in LDS for the ranking loops.
 constexpr uint32_t kRankTile = 2048;
 
-template <typename Ids>
-__device__ __forceinline__ uint32_t rank_in_cell(Ids ids, uint32_t c0, uint32_t c1, uint32_t myId) {
+// Rank of source slot `mine` among the source slots of its new cell, the
+// entries [c0, c1) of the bucketed slots: the canonical order within a cell is
+// the order of the slots (a stable counting sort of the slot order).
+template <typename Slots>
+__device__ __forceinline__ uint32_t rank_in_cell(Slots slots, uint32_t c0, uint32_t c1, uint32_t mine) {
   uint32_t rank = 0, q = c0;
   for (; q + 4u <= c1; q += 4u) {  // four independent loads in flight
-    const uint32_t a0 = ids(q), a1 = ids(q + 1), a2 = ids(q + 2), a3 = ids(q + 3);
-    rank += (a0 < myId ? 1u : 0u) + (a1 < myId ? 1u : 0u) + (a2 < myId ? 1u : 0u) + (a3 < myId ? 1u : 0u);
+    const uint32_t a0 = slots(q), a1 = slots(q + 1), a2 = slots(q + 2), a3 = slots(q + 3);
+    rank += (a0 < mine ? 1u : 0u) + (a1 < mine ? 1u : 0u) + (a2 < mine ? 1u : 0u) + (a3 < mine ? 1u : 0u);
   }
-  for (; q < c1; ++q) rank += (ids(q) < myId) ? 1u : 0u;
+  for (; q < c1; ++q) rank += (slots(q) < mine) ? 1u : 0u;
   return rank;
 }
 
 __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
-  __shared__ uint32_t tileIds[kRankTile];
+  __shared__ uint32_t tileSlots[kRankTile];
   __shared__ uint32_t span[2];
   const int p0 = (int)(blockIdx.x * kBlock);
   const int p = p0 + (int)threadIdx.x;
   const bool live = p < n;
-  uint32_t s = 0, myId = 0, key = 0, c0 = 0, c1 = 0;
+  uint32_t s = 0, key = 0, c0 = 0, c1 = 0;
   if (live) {
     s = b.tmpSlot[p];
-    myId = b.tmpId[p];
     key = b.keyp[s];
     c0 = b.startNext[key];
     c1 = b.startNext[key + 1];
@@ -494,16 +536,154 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
   const uint32_t lo = span[0], hi = span[1];
   const bool tiled = hi - lo <= kRankTile;
   if (tiled)
-    for (uint32_t q = lo + threadIdx.x; q < hi; q += kBlock) tileIds[q - lo] = b.tmpId[q];
+    for (uint32_t q = lo + threadIdx.x; q < hi; q += kBlock) tileSlots[q - lo] = b.tmpSlot[q];
   __syncthreads();
   if (!live) return;
-  const uint32_t rank = tiled ? rank_in_cell([&](uint32_t q) { return tileIds[q - lo]; }, c0, c1, myId)
-                              : rank_in_cell([&](uint32_t q) { return b.tmpId[q]; }, c0, c1, myId);
+  const uint32_t rank = tiled ? rank_in_cell([&](uint32_t q) { return tileSlots[q - lo]; }, c0, c1, s)
+                              : rank_in_cell([&](uint32_t q) { return b.tmpSlot[q]; }, c0, c1, s);
   const uint32_t d = c0 + rank;
   b.x1[d] = b.xp[s];
   b.prev[d] = b.x0[s];  // demo4.cpp:334 prevPosition = curPosition
-  b.id2[d] = myId;
+  b.id2[d] = b.id[s];
   b.keys[d] = key;
+}
+
+// ---------------------------------------------------------------------------
+// 3'. Placement: bucket, order and move in one pass (large grids, every
+// particle binned by the last step).
+//
+// Workgroup g produces the new slots [g*kPlaceChunk, (g+1)*kPlaceChunk): the
+// cells [cf, cl] that cover them (the chunk table the cell scan writes).  Its
+// sources are the particles whose new cell lies in [cf, cl]: those whose cell
+// of the last binning is 3x3-adjacent to it sit in three slot windows of the
+// last binning (the rows below, at and above [cf, cl], one cell wider on each
+// side), the others are in the far list.  They are bucketed by new cell in an
+// LDS histogram (the bucket of cell c starts at its new cell start), each is
+// ranked by its slot among its cell's bucket (the canonical order: a stable
+// counting sort of the slot order), and the ones whose rank lands in the
+// workgroup's range are moved.  A chunk whose cells hold more than kPlaceCap
+// particles ranks by scanning its sources instead (correct, slow; only for
+// cells of thousands of particles).
+// ---------------------------------------------------------------------------
+
+// Calls f(s, key) for every particle whose new cell key is in [cf, cl]; each
+// thread takes a strided share.  Window slots of far movers are skipped (they
+// come from the far list).
+template <bool MULTI, typename F>
+__device__ __forceinline__ void for_each_place_source(const Buffers &b, const DevScene &s0, uint32_t cf, uint32_t cl,
+                                                      uint32_t farN, F &&f) {
+  const int sFirst = MULTI ? scene_of_key(b.sceneBase, b.nScenes, cf) : 0;
+  const int sLast = MULTI ? scene_of_key(b.sceneBase, b.nScenes, cl) : 0;
+  for (int si = sFirst; si <= sLast; ++si) {
+    const DevScene sc = MULTI ? b.scenes[si] : s0;
+    const int64_t base = sc.cellBase, last = base + (int64_t)sc.nx * sc.ny - 1;
+    const int64_t a = max((int64_t)cf, base), e = min((int64_t)cl, last);
+    // key windows of the rows below, at and above [a, e], merged in key order
+    int64_t wa[3], wb[3];
+    int m = 0;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int64_t lo = max(a + (int64_t)dy * sc.nx - 1, base), hi = min(e + (int64_t)dy * sc.nx + 1, last);
+      if (lo > hi) continue;
+      if (m > 0 && lo <= wb[m - 1] + 1) {
+        wb[m - 1] = max(wb[m - 1], hi);
+      } else {
+        wa[m] = lo;
+        wb[m] = hi;
+        ++m;
+      }
+    }
+    for (int w = 0; w < m; ++w) {
+      const uint32_t s1 = b.startCur[wb[w] + 1];
+      // four independent key loads in flight per thread
+      for (uint32_t s = b.startCur[wa[w]] + threadIdx.x; s < s1; s += 4u * kBlock) {
+        uint32_t k[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) k[u] = (s + u * kBlock < s1) ? b.keyp[s + u * kBlock] : 0xFFFFFFFFu;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)  // far movers carry the flag and fall outside [cf, cl]
+          if (k[u] >= cf && k[u] <= cl) f(s + u * kBlock, k[u]);
+      }
+    }
+  }
+  for (uint32_t q = threadIdx.x; q < farN; q += kBlock) {
+    const uint32_t s = b.farList[q];
+    const uint32_t k = b.keyp[s] & ~kFarFlag;
+    if (k >= cf && k <= cl) f(s, k);
+  }
+}
+
+__device__ __forceinline__ void place_move(const Buffers &b, uint32_t s, uint32_t d, uint32_t k) {
+  b.x1[d] = b.xp[s];
+  b.prev[d] = b.x0[s];  // demo4.cpp:334 prevPosition = curPosition
+  b.id2[d] = b.id[s];
+  b.keys[d] = k;
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_place(Buffers b, DevScene s0, int n, uint32_t cells) {
+  __shared__ uint32_t bucket[kPlaceCap];   // source slot per new slot of [lo, hi)
+  __shared__ uint32_t bkey[kPlaceCap];     // its new cell
+  __shared__ uint32_t fillc[kPlaceCap];    // arrivals per cell, at the cell's first bucket entry
+  __shared__ uint32_t cstart[kPlaceCap + 1];  // new cell starts of [cf, cl + 1]
+  __shared__ uint32_t span[2];
+  const uint32_t g0 = blockIdx.x * kPlaceChunk, g1 = min(g0 + kPlaceChunk, (uint32_t)n);
+  if (threadIdx.x < 2) span[threadIdx.x] = b.chunkCells[2 * blockIdx.x + threadIdx.x];
+  __syncthreads();
+  const uint32_t cf = span[0], cl = span[1];
+  const uint32_t lo = b.startNext[cf], hi = b.startNext[cl + 1];
+  const uint32_t farN = min(*b.farCount, (uint32_t)n);
+  if (hi - lo <= kPlaceCap) {
+    const bool staged = cl - cf + 2u <= kPlaceCap + 1u;
+    for (uint32_t q = threadIdx.x; q < hi - lo; q += kBlock) {
+      fillc[q] = 0u;
+      bucket[q] = 0xFFFFFFFFu;
+    }
+    if (staged)
+      for (uint32_t q = threadIdx.x; q < cl - cf + 2u; q += kBlock) cstart[q] = b.startNext[cf + q];
+    __syncthreads();
+    auto start_of = [&](uint32_t k) { return staged ? cstart[k - cf] : b.startNext[k]; };
+    for_each_place_source<MULTI>(b, s0, cf, cl, farN, [&](uint32_t s, uint32_t k) {
+      const uint32_t c0 = start_of(k), count = start_of(k + 1) - c0;
+      const uint32_t a = atomicAdd(&fillc[c0 - lo], 1u);  // LDS histogram: arrival in the cell's bucket
+      if (a < count) {
+        bucket[c0 - lo + a] = s;
+        bkey[c0 - lo + a] = k;
+      }
+    });
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < hi - lo; q += kBlock) {
+      const uint32_t s = bucket[q];
+      if (s >= (uint32_t)n) continue;  // never, when every particle is found once
+      const uint32_t k = bkey[q];
+      const uint32_t c0 = start_of(k), c1 = start_of(k + 1);
+      if (c1 <= g0 || c0 >= g1) continue;  // a cell the neighbouring workgroup moves
+      const uint32_t d = c0 + rank_in_cell([&](uint32_t e) { return bucket[e - lo]; }, c0, c1, s);
+      if (d >= g0 && d < g1) place_move(b, s, d, k);
+    }
+    return;
+  }
+  // Oversized cells: rank each source against every source of its cell.
+  for_each_place_source<MULTI>(b, s0, cf, cl, farN, [&](uint32_t s, uint32_t k) {
+    const uint32_t c0 = b.startNext[k], c1 = b.startNext[k + 1];
+    if (c1 <= g0 || c0 >= g1) return;
+    uint32_t rank = 0;
+    const int si = MULTI ? scene_of_key(b.sceneBase, b.nScenes, k) : 0;
+    const DevScene sc = MULTI ? b.scenes[si] : s0;
+    const int64_t base = sc.cellBase, last = base + (int64_t)sc.nx * sc.ny - 1;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int64_t wlo = max((int64_t)k + (int64_t)dy * sc.nx - 1, base);
+      const int64_t whi = min((int64_t)k + (int64_t)dy * sc.nx + 1, last);
+      if (wlo > whi) continue;
+      const uint32_t t1 = min(b.startCur[whi + 1], s);
+      for (uint32_t t = b.startCur[wlo]; t < t1; ++t) rank += (b.keyp[t] == k) ? 1u : 0u;
+    }
+    for (uint32_t q = 0; q < farN; ++q) {
+      const uint32_t t = b.farList[q];
+      rank += (t < s && (b.keyp[t] & ~kFarFlag) == k) ? 1u : 0u;
+    }
+    const uint32_t d = c0 + rank;
+    if (d >= g0 && d < g1) place_move(b, s, d, k);
+  });
 }
 
 // Exclusive scan of the cell counts in one workgroup, for grids of at most
@@ -1097,10 +1277,10 @@ void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s
   }
 }
 
-void launch_scan(const Buffers &b, uint32_t cells, hipStream_t s) {
+void launch_scan(const Buffers &b, uint32_t cells, int n, hipStream_t s) {
   const uint32_t nb = (cells + kScanChunk - 1) / kScanChunk;
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, b, cells);
-  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kBlock), 0, s, b, cells);
+  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kBlock), 0, s, b, cells, (uint32_t)n);
 }
 
 void launch_scatter_reorder(const Buffers &b, int n, hipStream_t s) {
@@ -1109,14 +1289,32 @@ void launch_scatter_reorder(const Buffers &b, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_rank_reorder, dim3(grid_for(n)), dim3(kBlock), 0, s, b, n);
 }
 
-uint32_t launch_grid(const Buffers &b, uint32_t cells, int n, int viscCap, hipStream_t s) {
+bool place_binning(uint32_t cells, int viscCap, bool allBinned) {
+  return cells > kSmallGridCells && viscCap != kWideCap && allBinned;
+}
+
+uint32_t launch_grid(const Buffers &b, const DevScene &s0, uint32_t cells, int n, int viscCap, bool place,
+                     hipStream_t s) {
   uint32_t slots = 1u;
+  if (place) {
+    launch_scan(b, cells, n, s);
+    if (n > 0) {
+      const unsigned chunks = (unsigned)(((uint64_t)n + kPlaceChunk - 1) / kPlaceChunk);
+      if (b.nScenes > 1)
+        hipLaunchKernelGGL(k_place<true>, dim3(chunks), dim3(kBlock), 0, s, b, s0, n, cells);
+      else
+        hipLaunchKernelGGL(k_place<false>, dim3(chunks), dim3(kBlock), 0, s, b, s0, n, cells);
+    }
+    return (cells + kScanChunk - 1) / kScanChunk;
+  }
   if (count_in_grid(n, cells, viscCap)) {
     hipLaunchKernelGGL(k_count_scan_small, dim3(1), dim3(kSmallGridBlock), 0, s, b, n, cells);
   } else if (cells <= kSmallGridCells) {
     hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kSmallGridBlock), 0, s, b, cells);
   } else {
-    launch_scan(b, cells, s);
+    Buffers bs = b;
+    bs.chunkCells = nullptr;
+    launch_scan(bs, cells, n, s);
     slots = (cells + kScanChunk - 1) / kScanChunk;
   }
   launch_scatter_reorder(b, n, s);

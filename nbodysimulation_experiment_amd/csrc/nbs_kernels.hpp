@@ -56,8 +56,10 @@ struct DevBody {
 __host__ __device__ inline int float_order(int bits) { return bits ^ ((bits >> 31) & 0x7fffffff); }
 
 // Device storage.  Particle arrays are indexed by slot: slots [0, nBinned)
-// hold the particles of the last binning sorted by (cell key, particle index);
-// slots [nBinned, n) hold particles added since, in index order.
+// hold the particles of the last binning sorted by cell key, each cell in the
+// order of the particles' slots before that binning (a stable counting sort
+// of the slot order); slots [nBinned, n) hold particles added since, in index
+// order.
 struct Buffers {
   float2 *x0;    // positions (after the last step)
   float2 *xp;    // predicted positions, slot order of the previous binning
@@ -71,7 +73,7 @@ struct Buffers {
   uint32_t *keys;      // cell key of the last binning per slot
   uint32_t *keyp;      // cell key of the predicted position (previous slot order)
   uint32_t *fill;      // arrival index within the new cell
-  uint32_t *tmpSlot, *tmpId;  // cell-bucketed slots and particle indices
+  uint32_t *tmpSlot;   // source slots bucketed by new cell
   uint32_t *scn;       // scene of particles not yet binned
   uint32_t *count;     // per cell, zero between steps
   uint32_t *startCur;  // per cell + 1: the last binning
@@ -86,7 +88,17 @@ struct Buffers {
   const float2 *verts;
   float2 *dxOut;       // if set, the relaxation kernel also writes each particle's
                        // relaxation displacement (before collisions) here
+  // Placement mode of the grid phase (launch_grid with place = true): the
+  // viscosity kernel flags (bit 31 of keyp) and lists here the particles whose
+  // new cell is not 3x3-adjacent to their cell of the last binning.
+  uint32_t *farList;
+  uint32_t *farCount;       // entries of farList this step (zero when the step starts)
+  uint32_t *farCountClear;  // the other step parity's counter: cleared by the cell scan
+  uint32_t *chunkCells;     // per kPlaceChunk new slots: the cells holding its first and last slot
 };
+
+constexpr uint32_t kPlaceChunk = 1024;  // new slots per placement workgroup
+constexpr uint32_t kPlaceCap = 2048;    // LDS bucket entries of a placement workgroup
 
 // Launchers (nbs_kernels.hip).  All kernels are enqueued on `stream`.
 void launch_unbin(const Buffers &b, int nBinned, hipStream_t stream);
@@ -102,12 +114,19 @@ void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s
 // (viscCap == kWideCap), at most kSmallGridCells cells and
 // kSmallCountParticles particles.
 bool count_in_grid(int n, uint32_t cells, int viscCap);
-void launch_scan(const Buffers &b, uint32_t cells, hipStream_t stream);
+// n: particles (with b.chunkCells set, the scan also fills the chunk table).
+void launch_scan(const Buffers &b, uint32_t cells, int n, hipStream_t stream);
 void launch_scatter_reorder(const Buffers &b, int n, hipStream_t stream);
 // The grid phase: the cell scan (one workgroup for small grids, else
-// launch_scan), then launch_scatter_reorder.  Returns the number of (min, max)
-// occupied-cell pairs it leaves in cellStats.
-uint32_t launch_grid(const Buffers &b, uint32_t cells, int n, int viscCap, hipStream_t stream);
+// launch_scan), then launch_scatter_reorder, or with place = true the cell scan
+// and k_place (b.farList etc. set, see place_binning).  Returns the number of
+// (min, max) occupied-cell pairs it leaves in cellStats.
+uint32_t launch_grid(const Buffers &b, const DevScene &s0, uint32_t cells, int n, int viscCap, bool place,
+                     hipStream_t stream);
+// Whether a step can bin by placement (k_place): a grid too large for the
+// one-workgroup scan, the one-lane-per-particle viscosity kernel (which flags
+// the far movers), and every particle binned by the last step.
+bool place_binning(uint32_t cells, int viscCap, bool allBinned);
 void launch_density(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int cap, hipStream_t stream);
 // Workgroups of the density launch for n particles at capacity cap: the number
 // of (min, max) candidate-count pairs it leaves in candStats.

@@ -712,7 +712,8 @@ class Simulation {
     grow2(d_.x0); grow2(d_.prev); grow2(d_.v); grow2(d_.dens); grow2(d_.pres); grow2(d_.acc);
     grow4(d_.id); grow4(d_.keys); grow4(d_.scn);
     d_.xp.alloc(cap); d_.x1.alloc(cap); d_.id2.alloc(cap); d_.keyp.alloc(cap); d_.fill.alloc(cap);
-    d_.tmpSlot.alloc(cap); d_.tmpId.alloc(cap);
+    d_.tmpSlot.alloc(cap); d_.farList.alloc(cap);
+    d_.chunkCells.alloc(2 * ((cap + kPlaceChunk - 1) / kPlaceChunk));
     d_.candStats.alloc(2 * (size_t)density_stat_slots((int)cap, kWideCap));
     cap_ = cap;
     rebindBuffers();
@@ -726,6 +727,8 @@ class Simulation {
     d_.startNext.alloc(c + 1);
     d_.blockSum.alloc(std::max<size_t>(nb, 1));
     d_.cellStats.alloc(2 * std::max<size_t>(nb, 1));
+    if (!d_.farCounts.p) d_.farCounts.alloc(2);
+    launch_fill_u32(d_.farCounts.p, 0u, 2, stream_);
     launch_fill_u32(d_.count.p, 0u, c, stream_);
     launch_fill_u32(d_.startCur.p, 0u, c + 1, stream_);
     launch_fill_u32(d_.startNext.p, 0u, c + 1, stream_);
@@ -755,7 +758,8 @@ class Simulation {
     buf_.x0 = d_.x0.p; buf_.xp = d_.xp.p; buf_.x1 = d_.x1.p; buf_.prev = d_.prev.p; buf_.v = d_.v.p;
     buf_.dens = d_.dens.p; buf_.pres = d_.pres.p; buf_.acc = d_.acc.p;
     buf_.id = d_.id.p; buf_.id2 = d_.id2.p; buf_.keys = d_.keys.p; buf_.keyp = d_.keyp.p; buf_.fill = d_.fill.p;
-    buf_.tmpSlot = d_.tmpSlot.p; buf_.tmpId = d_.tmpId.p; buf_.scn = d_.scn.p;
+    buf_.tmpSlot = d_.tmpSlot.p; buf_.scn = d_.scn.p;
+    buf_.farList = nullptr; buf_.farCount = buf_.farCountClear = nullptr; buf_.chunkCells = nullptr;
     buf_.count = d_.count.p; buf_.startCur = d_.startCur.p; buf_.startNext = d_.startNext.p;
     buf_.blockSum = d_.blockSum.p;
     buf_.candStats = d_.candStats.p; buf_.cellStats = d_.cellStats.p;
@@ -857,10 +861,19 @@ class Simulation {
     const int n = (int)n_;
     int cap[3];
     pairCaps(cap);
+    // Placement binning: the viscosity kernel lists the far movers, k_place
+    // orders and moves; on large grids every step's scan clears the counter
+    // of the next step's parity.
+    const bool place = place_binning(totalCells_, cap[0], lists_ && nBinned_ == n_);
+    Buffers bg = buf_;
+    bg.farList = place ? d_.farList.p : nullptr;
+    bg.chunkCells = place ? d_.chunkCells.p : nullptr;
+    bg.farCount = d_.farCounts.p + parity_;
+    bg.farCountClear = d_.farCounts.p + (parity_ ^ 1u);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 0], stream_));
-    launch_visc_predict(buf_, p, s0, n, (int)nBinned_, lists_ ? 1 : 0, cap[0], totalCells_, stream_);
+    launch_visc_predict(bg, p, s0, n, (int)nBinned_, lists_ ? 1 : 0, cap[0], totalCells_, stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 1], stream_));
-    cellSlots_ = launch_grid(buf_, totalCells_, n, cap[0], stream_);
+    cellSlots_ = launch_grid(bg, s0, totalCells_, n, cap[0], place, stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 2], stream_));
     launch_density(buf_, p, s0, n, cap[1], stream_);
     statSlots_ = density_stat_slots(n, cap[1]);
@@ -955,6 +968,7 @@ class Simulation {
   }
 
   void swapAfterStep() {
+    parity_ ^= 1u;
     std::swap(buf_.id, buf_.id2);
     d_.id.swap(d_.id2);
     std::swap(buf_.startCur, buf_.startNext);
@@ -1311,7 +1325,9 @@ class Simulation {
 
   struct Storage {
     DevBuf<float2> x0, xp, x1, prev, v, dens, pres, acc;
-    DevBuf<uint32_t> id, id2, keys, keyp, fill, tmpSlot, tmpId, scn, count, startCur, startNext, blockSum;
+    DevBuf<uint32_t> id, id2, keys, keyp, fill, tmpSlot, scn, count, startCur, startNext, blockSum;
+    DevBuf<uint32_t> farList, farCounts;  // placement binning: far movers, one counter per step parity
+    DevBuf<uint32_t> chunkCells;          // placement binning: first and last cell of each chunk
     DevBuf<uint32_t> candStats, cellStats;
   };
 
@@ -1340,6 +1356,7 @@ class Simulation {
 
   uint64_t n_ = 0, nBinned_ = 0, cap_ = 0;
   bool lists_ = false;
+  uint32_t parity_ = 0;  // step parity: selects the far-mover counter of placement binning
 
   nbs_stats stats_h_{};
   bool statsPending_ = false;

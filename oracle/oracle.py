@@ -30,6 +30,7 @@ MUTATIONS = {
     "velocity_no_inv_dt": 8,
     "circle_no_radius": 9,
     "skip_self": 10,
+    "grid_by_index": 11,
 }
 
 

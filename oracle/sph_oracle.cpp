@@ -462,12 +462,16 @@ struct GSSim : SimBase {
 };
 
 // ---------------------------------------------------------------------------
-// Jacobi order with the canonical (cell key, particle id) grid.
+// Jacobi order with the canonical grid: a stable counting sort of the slot
+// order by cell key (see bin()).
 // ---------------------------------------------------------------------------
 template <typename T>
 struct JacobiSim : SimBase {
   World<T> w;
   std::vector<V2<T>> x, prev, v, a, dens, pres, tmp;
+  // order: the slot order, a permutation of the particle indices.  Its first
+  // nBinned entries are the last binning's cell lists; particles added since
+  // follow in index order.
   std::vector<uint32_t> scene, key, order, cellStart;
   int64_t nBinned = 0;  // particles present at the last binning
   Stats st;
@@ -512,6 +516,7 @@ struct JacobiSim : SimBase {
       pres.push_back({T(0), T(0)});
       scene.push_back((uint32_t)sc);
       key.push_back(cellKey(w.scenes[sc], p));
+      order.push_back((uint32_t)(first + k));
     }
     return first;
   }
@@ -521,6 +526,7 @@ struct JacobiSim : SimBase {
       v[i] = {T(vs[2 * i]), T(vs[2 * i + 1])};
       a[i] = {T(0), T(0)};
       key[i] = cellKey(w.scenes[scene[i]], x[i]);
+      order[i] = (uint32_t)i;  // the slot order restarts in index order
     }
     nBinned = 0;  // a restored state starts without neighbour lists
   }
@@ -540,8 +546,14 @@ struct JacobiSim : SimBase {
       }
     });
   }
-  // Counting sort of ids 0..n-1 by cell key; ids are visited in ascending
-  // order, so each cell lists its particles by ascending id.
+  // Stable counting sort of the slot order by cell key: each cell lists its
+  // particles in slot order, i.e. by their cell of the last binning and their
+  // place in it, then the particles added since by index.  The slot order
+  // starts in index order (and restarts there after setState), so a first
+  // binning lists each cell by ascending index, the reference's order after
+  // AddParticle (demo4.cpp:46-47).  The reference's later per-cell order is
+  // its swap-remove history (demo4.cpp:56-76), which no parallel re-binning
+  // reproduces; this order only keeps the candidate sets.
   void bin() {
     const int64_t n = (int64_t)x.size();
     std::fill(cellStart.begin(), cellStart.end(), 0u);
@@ -554,9 +566,13 @@ struct JacobiSim : SimBase {
     st.maxCellParticleCount = mx;
     st.minCellParticleCount = mn;
     for (uint32_t k = 0; k < w.cells; ++k) cellStart[k + 1] += cellStart[k];
-    order.assign((size_t)n, 0u);
+    std::vector<uint32_t> next((size_t)n, 0u);
     std::vector<uint32_t> fill(cellStart.begin(), cellStart.end() - 1);
-    for (int64_t i = 0; i < n; ++i) order[fill[key[i]]++] = (uint32_t)i;
+    for (int64_t q = 0; q < n; ++q) {
+      const uint32_t i = g_mutation == MUT_GRID_BY_INDEX ? (uint32_t)q : order[q];
+      next[fill[key[i]]++] = i;
+    }
+    order.swap(next);
     nBinned = n;
   }
 

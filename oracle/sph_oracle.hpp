@@ -59,6 +59,7 @@ enum Mutation : int {
   MUT_VELOCITY_NO_INV_DT = 8,  // velocity = x - prev (no 1/dt)
   MUT_CIRCLE_NO_RADIUS = 9,    // circle contact radius ignores the particle radius
   MUT_SKIP_SELF = 10,          // candidate list drops the particle itself
+  MUT_GRID_BY_INDEX = 11,      // Jacobi grid sorts cells by particle index, not by slot order
   MUT_COUNT
 };
 extern int g_mutation;
