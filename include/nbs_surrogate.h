@@ -10,7 +10,7 @@
  * r^2 < h^2) with R = x_j - x_i, v = (x - prev) / dt, q = 1 - r/h:
  *   f_ij = (R/h, (v_j - v_i) sv, (rho_i - mr) sr, (rhon_i - mn) sn,
  *           (rho_j - mr) sr, (rhon_j - mn) sn, 1, q, q^2, 0 x 5)    16 features
- *   H_i = sum_j relu(W1 f_ij);  M_i = W2 H_i + n_i b2            message net 16-64-64
+ *   H_i = sum_j bf16(relu(W1 f_ij)); M_i = W2 H_i + n_i b2       message net 16-64-64
  *   G_i = relu(W3 M_i + b3);    o_i = W4 G_i + b4                head 64-64-2
  *   x_i = collide(x1_i + o_i os),  v_i = (x_i - prev_i) / dt
  * Parameter vector (FP32, NBS_SURROGATE_PARAM_COUNT floats): W1 (64 x 16,
@@ -95,9 +95,11 @@ nbs_status nbs_surrogate_get_report(nbs_sim *sim, nbs_surrogate_report *report);
 nbs_status nbs_surrogate_frame(nbs_sim *sim, float dt, float *grad, float *loss, float *out, float *x, float *prev,
                                float *density, float *target, int32_t *scene);
 
-/* MFMA self-test: c (32 x 32) = a (32 x 8) b (8 x 32), row-major FP32 inputs
- * rounded to BF16 and multiplied by one v_mfma_f32_32x32x8_bf16. */
+/* MFMA self-tests: c (32 x 32) = a (32 x K) b (K x 32), row-major FP32 inputs
+ * rounded to BF16 and multiplied by one v_mfma_f32_32x32x8_bf16 (K = 8) or
+ * one v_mfma_f32_32x32x16_bf16 (K = 16, the form the network's kernels use). */
 nbs_status nbs_selftest_mfma(const float *a, const float *b, float *c);
+nbs_status nbs_selftest_mfma16(const float *a, const float *b, float *c);
 
 #ifdef __cplusplus
 }

@@ -136,6 +136,7 @@ SIGNATURES = {
     "nbs_surrogate_get_scene_errors": [vp, vp, c_uint64],
     "nbs_surrogate_frame": [vp, c_float, vp, vp, vp, vp, vp, vp, vp, vp],
     "nbs_selftest_mfma": [vp, vp, vp],
+    "nbs_selftest_mfma16": [vp, vp, vp],
 }
 
 

@@ -463,6 +463,8 @@ class Simulation {
     S.w1b.alloc(kHid * kFeat);
     S.w2b.alloc(kHid * kHid);
     S.w3b.alloc(kHid * kHid);
+    S.w2t.alloc(kHid * kHid);
+    S.w3t.alloc(kHid * kHid);
     std::vector<float> h(kNumParams, 0.0f);
     uint64_t draw = 0;
     auto fill = [&](int off, int count, float bound) {
@@ -1083,7 +1085,7 @@ class Simulation {
 
   struct SurrogateState {
     DevBuf<float> params, adamM, adamV, grad, out, act, cnt, dH, dG, dM, dO, part, lossPart, loss;
-    DevBuf<uint16_t> w1b, w2b, w3b;
+    DevBuf<uint16_t> w1b, w2b, w3b, w2t, w3t;
     DevBuf<float2> target, shadowX, shadowV, densA, densB, presScratch;
     DevBuf<uint32_t> candScratch, pairPart;
     DevBuf<double> errPartials, errOut, sceneErr;
@@ -1109,7 +1111,7 @@ class Simulation {
   void uploadParams(const float *h) {
     SurrogateState &S = *sur_;
     NBS_HIP(hipMemcpyAsync(S.params.p, h, kNumParams * 4, hipMemcpyHostToDevice, stream_));
-    launch_bf16_weights(S.params.p, S.w1b.p, S.w2b.p, S.w3b.p, stream_);
+    launch_bf16_weights(S.params.p, S.w1b.p, S.w2b.p, S.w3b.p, S.w2t.p, S.w3t.p, stream_);
     NBS_HIP(hipGetLastError());
     NBS_HIP(hipStreamSynchronize(stream_));
   }
@@ -1158,6 +1160,8 @@ class Simulation {
     m.w1b = S.w1b.p;
     m.w2b = S.w2b.p;
     m.w3b = S.w3b.p;
+    m.w2t = S.w2t.p;
+    m.w3t = S.w3t.p;
     m.pairPart = S.pairPart.p;
     if (training) {
       m.out = S.out.p;
@@ -1214,7 +1218,7 @@ class Simulation {
           launch_surrogate_backward(buf_, p, s0, n, m, stream_);
           launch_adam(S.params.p, S.grad.p, S.adamM.p, S.adamV.p, kNumWeights, S.onlineLr, 0.9f, 0.999f, 1.0e-8f,
                       (int)++S.adamT, stream_);
-          launch_bf16_weights(S.params.p, S.w1b.p, S.w2b.p, S.w3b.p, stream_);
+          launch_bf16_weights(S.params.p, S.w1b.p, S.w2b.p, S.w3b.p, S.w2t.p, S.w3t.p, stream_);
           ++S.rep.train_steps;
         }
         break;
@@ -1232,7 +1236,7 @@ class Simulation {
         if (relax == RELAX_TRAIN) {
           launch_adam(S.params.p, S.grad.p, S.adamM.p, S.adamV.p, kNumWeights, S.lr, 0.9f, 0.999f, 1.0e-8f,
                       (int)++S.adamT, stream_);
-          launch_bf16_weights(S.params.p, S.w1b.p, S.w2b.p, S.w3b.p, stream_);
+          launch_bf16_weights(S.params.p, S.w1b.p, S.w2b.p, S.w3b.p, S.w2t.p, S.w3t.p, stream_);
           ++S.rep.train_steps;
         }
         break;
@@ -1811,6 +1815,23 @@ nbs_status nbs_selftest_mfma(const float *a, const float *b, float *c) {
     NBS_HIP(hipMemcpy(da.p, a, 32 * 8 * 4, hipMemcpyHostToDevice));
     NBS_HIP(hipMemcpy(db.p, b, 8 * 32 * 4, hipMemcpyHostToDevice));
     nbs::launch_mfma_probe(da.p, db.p, dc.p, nullptr);
+    NBS_HIP(hipGetLastError());
+    NBS_HIP(hipMemcpy(c, dc.p, 32 * 32 * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+nbs_status nbs_selftest_mfma16(const float *a, const float *b, float *c) {
+  if (!a || !b || !c) return NBS_ERR_INVALID_ARGUMENT;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return NBS_ERR_NO_DEVICE;
+  return guarded([&] {
+    nbs::DevBuf<float> da, db, dc;
+    da.alloc(32 * 16);
+    db.alloc(16 * 32);
+    dc.alloc(32 * 32);
+    NBS_HIP(hipMemcpy(da.p, a, 32 * 16 * 4, hipMemcpyHostToDevice));
+    NBS_HIP(hipMemcpy(db.p, b, 16 * 32 * 4, hipMemcpyHostToDevice));
+    nbs::launch_mfma16_probe(da.p, db.p, dc.p, nullptr);
     NBS_HIP(hipGetLastError());
     NBS_HIP(hipMemcpy(c, dc.p, 32 * 32 * 4, hipMemcpyDeviceToHost));
   });

@@ -1,18 +1,27 @@
 // Learned relaxation kernels (BF16 MFMA, FP32 accumulation) for gfx950.
 //
-// v_mfma_f32_32x32x8_bf16 operand layouts (lane l, g = l / 32, c = l % 32):
-//   A (32 x 8):  lane holds A[c][4g + t], t = 0..3
-//   B (8 x 32):  lane holds B[4g + t][c]
+// v_mfma_f32_32x32x16_bf16 operand layouts (lane l, g = l / 32, c = l % 32):
+//   A (32 x 16): lane holds A[c][8g + t], t = 0..7
+//   B (16 x 32): lane holds B[8g + t][c]
 //   C (32 x 32): register r holds C[8(r/4) + 4g + (r%4)][c]
-// The MFMA self-test checks this layout against a host matrix product.
+// (v_mfma_f32_32x32x8_bf16, the layout self-test's other form: A[c][4g + t],
+// B[4g + t][c], t = 0..3, the same C.)  The MFMA self-tests check both
+// layouts against host matrix products.
 //
 // Pair layer: one wave serves 64 particles, one per lane.  Each lane walks its
 // own particle's candidates (the three row ranges of its 3x3 cells) and stops
 // at its next in-range pair; the wave's 64 current pairs are two 32-row MFMA
-// tiles (rows = lanes 0..31 and lanes 32..63).  A lane without a pair left
+// tiles (rows = lanes 0..31 and lanes 32..63), one K = 16 MFMA per tile and
+// hidden half gives the pre-activations z.  A lane without a pair left
 // contributes a zero feature row, whose messages are relu(0) = 0 because the
-// bias is a feature.  The relu'd pre-activations accumulate per row in the C
-// layout, which after the last pair holds H for the wave's 64 particles.
+// bias is a feature.  relu(z) is taken on BF16: two values are converted with
+// one v_cvt_pk_bf16_f32 and clamped at 0 with one v_pk_max_i16 (a negative
+// BF16 is a negative 16-bit integer).  The per-particle sum H_i += relu(z_ij)
+// runs on the matrix core: in the C layout, registers 8s .. 8s + 7 of a lane
+// hold exactly the B operand (K = 16 rows of z, the lane's hidden column) of
+// a K = 16 MFMA, and a 0/1 selection matrix as A sends row k of z to its
+// particle's row of H; with one nonzero product per output, the MFMA adds
+// each BF16 value to the FP32 accumulator exactly as an FP32 add would.
 #include "nbs_device.hpp"
 #include "nbs_surrogate.hpp"
 
@@ -68,8 +77,67 @@ __device__ __forceinline__ bf16x4 load_wb(const uint16_t *W, int K, int n0, int 
 // here (the IEEE-mode canonicalisation of its operand).
 __device__ __forceinline__ float relu(float z) { return __int_as_float(max(__float_as_int(z), 0)); }
 
+// K = 16 operands: eight BF16 values in four words.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ bf16x8 bf8(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3) {
+  const u32x4 u = {w0, w1, w2, w3};
+  return __builtin_bit_cast(bf16x8, u);
+}
+
+__device__ __forceinline__ floatx16 mfma16(bf16x8 a, bf16x8 b, floatx16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+// relu of two values rounded to BF16 (round to nearest even), packed: one
+// v_cvt_pk_bf16_f32, then one v_pk_max_i16 with 0 (negative values and -0 are
+// negative 16-bit integers and become +0).
+__device__ __forceinline__ uint32_t relu2_bf16(float lo, float hi) {
+  const s16x2 v = __builtin_bit_cast(s16x2, pack2(lo, hi));
+  const s16x2 z = {0, 0};
+  return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, z));
+}
+
+// K = 16 B operand of a row-major BF16 matrix W (rows = outputs, K inputs):
+// B[k][n] = W[n0 + n][k0 + k].  K and k0 are multiples of 8 (16-byte loads).
+__device__ __forceinline__ bf16x8 load_wb16(const uint16_t *W, int K, int n0, int k0) {
+  const int c = threadIdx.x & 31, g = (threadIdx.x >> 5) & 1;
+  const uint4 w = *reinterpret_cast<const uint4 *>(W + (size_t)(n0 + c) * K + k0 + 8 * g);
+  return bf8(w.x, w.y, w.z, w.w);
+}
+
+// Eight BF16 values at p (a row of an LDS buffer, 4-byte aligned).
+__device__ __forceinline__ bf16x8 lds_bf8(const uint16_t *p) {
+  const uint32_t *w = reinterpret_cast<const uint32_t *>(p);
+  return bf8(w[0], w[1], w[2], w[3]);
+}
+
+// B operand of step s (K = 16 rows of a 32-row C-layout tile z): this lane's
+// registers 8s .. 8s + 7, relu'd in BF16.
+__device__ __forceinline__ bf16x8 relu_rows(const floatx16 &z, int s) {
+  return bf8(relu2_bf16(z[8 * s + 0], z[8 * s + 1]), relu2_bf16(z[8 * s + 2], z[8 * s + 3]),
+             relu2_bf16(z[8 * s + 4], z[8 * s + 5]), relu2_bf16(z[8 * s + 6], z[8 * s + 7]));
+}
+
 // C-layout row of register r for lane group g.
 __device__ __forceinline__ int c_row(int r, int g) { return 8 * (r >> 2) + 4 * g + (r & 3); }
+
+// A operand of step s of the pair sum: 32 x 16, A[p][k] = 1 when the K index k
+// (K = 16 rows of z as relu_rows supplies them: lane group g', register
+// 8s + t for k = 8g' + t) is row p of the tile.  Lane (c, g) holds row c.
+__device__ __forceinline__ bf16x8 selection(int s) {
+  const int c = threadIdx.x & 31, g = (threadIdx.x >> 5) & 1;
+  uint32_t w[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t lo = c_row(8 * s + 2 * q, g) == c ? 0x3F80u : 0u;  // BF16 1.0
+    const uint32_t hi = c_row(8 * s + 2 * q + 1, g) == c ? 0x3F80u : 0u;
+    w[q] = lo | (hi << 16);
+  }
+  return bf8(w[0], w[1], w[2], w[3]);
+}
 
 constexpr int kWaveRows = 64;    // particles per wave in the pair kernels
 constexpr int kHbStride = 68;    // BF16 per row of the head operand buffer (padded)
@@ -241,41 +309,36 @@ __device__ __forceinline__ void pair_words(uint32_t w[8], const PairSrc &src, co
 }
 
 struct W1Operands {
-  bf16x4 lo0, lo1, hi0, hi1;  // hidden 0..31 / 32..63, K steps 0 / 1
+  bf16x8 lo, hi;  // hidden 0..31 / 32..63
 };
 
 __device__ __forceinline__ W1Operands load_w1(const uint16_t *w1b) {
-  return {load_wb(w1b, kFeat, 0, 0), load_wb(w1b, kFeat, 0, 8), load_wb(w1b, kFeat, 32, 0),
-          load_wb(w1b, kFeat, 32, 8)};
+  return {load_wb16(w1b, kFeat, 0, 0), load_wb16(w1b, kFeat, 32, 0)};
 }
 
-// A operands of the two row tiles from each lane's own feature words: tile 0
-// rows are lanes 0..31, tile 1 rows lanes 32..63; a lane of group g supplies
-// features 8s + 4g .. 8s + 4g + 3, so half of the words cross the lane halves.
+// A operands (K = 16) of the two row tiles from each lane's own feature words:
+// tile 0 rows are lanes 0..31, tile 1 rows lanes 32..63; lane (c, g) supplies
+// features 8g .. 8g + 7 of its row, so one half of each tile's words crosses
+// the lane halves (words 6, 7 are zero).
 struct PairA {
-  bf16x4 t0s0, t0s1, t1s0, t1s1;  // tile, K step
+  bf16x8 t0, t1;
 };
 
 __device__ __forceinline__ PairA pair_operands(const uint32_t w[8]) {
   const int g = (threadIdx.x >> 5) & 1;
-  // step 0: features 0..7 (words 0..3)
-  const uint32_t s0 = g ? w[0] : w[2], s1 = g ? w[1] : w[3];
+  const uint32_t s0 = g ? w[0] : w[4], s1 = g ? w[1] : w[5], s2 = g ? w[2] : 0u, s3 = g ? w[3] : 0u;
   const uint32_t r0 = (uint32_t)__shfl_xor((int)s0, 32, 64), r1 = (uint32_t)__shfl_xor((int)s1, 32, 64);
-  // step 1: features 8..15 (words 4..7; words 6, 7 are zero)
-  const uint32_t q0 = (uint32_t)__shfl_xor((int)w[4], 32, 64), q1 = (uint32_t)__shfl_xor((int)w[5], 32, 64);
+  const uint32_t r2 = (uint32_t)__shfl_xor((int)s2, 32, 64), r3 = (uint32_t)__shfl_xor((int)s3, 32, 64);
   PairA a;
-  a.t0s0 = g ? words_to_bf4(r0, r1) : words_to_bf4(w[0], w[1]);
-  a.t1s0 = g ? words_to_bf4(w[2], w[3]) : words_to_bf4(r0, r1);
-  a.t0s1 = g ? words_to_bf4(0u, 0u) : words_to_bf4(w[4], w[5]);
-  a.t1s1 = g ? words_to_bf4(0u, 0u) : words_to_bf4(q0, q1);
+  a.t0 = g ? bf8(r0, r1, r2, r3) : bf8(w[0], w[1], w[2], w[3]);
+  a.t1 = g ? bf8(w[4], w[5], 0u, 0u) : bf8(r0, r1, r2, r3);
   return a;
 }
 
 // Pre-activations z = F W1^T of one row tile (C layout, lane = hidden unit).
-__device__ __forceinline__ void pair_layer(bf16x4 as0, bf16x4 as1, const W1Operands &w, floatx16 &zlo,
-                                           floatx16 &zhi) {
-  zlo = mfma(as1, w.lo1, mfma(as0, w.lo0, zero16()));
-  zhi = mfma(as1, w.hi1, mfma(as0, w.hi0, zero16()));
+__device__ __forceinline__ void pair_layer(bf16x8 a, const W1Operands &w, floatx16 &zlo, floatx16 &zhi) {
+  zlo = mfma16(a, w.lo, zero16());
+  zhi = mfma16(a, w.hi, zero16());
 }
 
 // ---------------------------------------------------------------------------
@@ -297,9 +360,10 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
   const float *prm = m.params;
   const float *norm = prm + kOffNorm;
   const W1Operands w1 = load_w1(m.w1b);
+  const bf16x8 sel[2] = {selection(0), selection(1)};
   const PairSrc src = stage_pairs<MULTI>(b, s0, base, n, pool, plan);
 
-  // 1. pair layer: H_i = sum_j relu(W1 f_ij)
+  // 1. pair layer: H_i = sum_j relu(W1 f_ij), each term rounded to BF16
   PartInfo q{};
   Cursor cu{0u, 0u, 0u, 0, true, 0u, 0u};
   if (me < n) {
@@ -321,17 +385,17 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
     }
     const PairA a = pair_operands(w);
     floatx16 zlo, zhi;
-    pair_layer(a.t0s0, a.t0s1, w1, zlo, zhi);
+    pair_layer(a.t0, w1, zlo, zhi);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      h00[k] += relu(zlo[k]);
-      h01[k] += relu(zhi[k]);
+    for (int st = 0; st < 2; ++st) {
+      h00 = mfma16(sel[st], relu_rows(zlo, st), h00);
+      h01 = mfma16(sel[st], relu_rows(zhi, st), h01);
     }
-    pair_layer(a.t1s0, a.t1s1, w1, zlo, zhi);
+    pair_layer(a.t1, w1, zlo, zhi);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      h10[k] += relu(zlo[k]);
-      h11[k] += relu(zhi[k]);
+    for (int st = 0; st < 2; ++st) {
+      h10 = mfma16(sel[st], relu_rows(zlo, st), h10);
+      h11 = mfma16(sel[st], relu_rows(zhi, st), h11);
     }
   }
   shCnt[lane] = (float)np;
@@ -344,18 +408,15 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
 
   // 2. head, one 32-row tile at a time: C = A W^T with A in hb (BF16)
   auto dense = [&](const uint16_t *W, floatx16 &olo, floatx16 &ohi) {
-    bf16x4 a[8];
+    bf16x8 a[4];
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      const uint16_t *src = &hb[c][8 * s + 4 * g];
-      a[s] = words_to_bf4((uint32_t)src[0] | ((uint32_t)src[1] << 16), (uint32_t)src[2] | ((uint32_t)src[3] << 16));
-    }
+    for (int s = 0; s < 4; ++s) a[s] = lds_bf8(&hb[c][16 * s + 8 * g]);
     olo = zero16();
     ohi = zero16();
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      olo = mfma(a[s], load_wb(W, kHid, 0, 8 * s), olo);
-      ohi = mfma(a[s], load_wb(W, kHid, 32, 8 * s), ohi);
+    for (int s = 0; s < 4; ++s) {
+      olo = mfma16(a[s], load_wb16(W, kHid, 0, 16 * s), olo);
+      ohi = mfma16(a[s], load_wb16(W, kHid, 32, 16 * s), ohi);
     }
   };
   const bool train = (mode & kFwdTrain) != 0;
@@ -437,71 +498,96 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
 // ---------------------------------------------------------------------------
 // Backward
 // ---------------------------------------------------------------------------
-// Per particle (FP32, fused multiply-adds): dO, dG (pre-activation), dM, dH,
-// the squared error and the squared normalised target.  The loops are unrolled
-// so the per-particle vectors stay in registers; the weights are wave-uniform
-// (scalar) loads.
-__global__ __launch_bounds__(kBlock) void k_sur_back_head(int n, SurrogateBufs m, float invN) {
-  __shared__ float red[2][kBlock / 64];
-  const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
-  float err = 0.0f, ref = 0.0f;
-  if (i < n) {
-    const float *prm = m.params;
-    const float os = prm[kOffNorm + 5];
-    const float *G = m.act + (size_t)i * kActStride + 2 * kHid;
+// Head backward, one wave per 32 particles (the rows of one tile), on the
+// matrix core: dO (FP32), dG = (dO W4) . [G > 0] (FP32), dM = dG W3 and
+// dH = dM W2 as K = 16 BF16 MFMAs (dG, dM and the weights rounded to BF16,
+// FP32 accumulation; the B operands read the transposed BF16 weight copies),
+// and the wave's squared error and squared normalised target.  dG, dM and dH
+// are written in FP32 for the weight gradients and the pair backward.
+constexpr int kHeadRows = 32;
+__global__ __launch_bounds__(64) void k_sur_back_head(int n, SurrogateBufs m, float invN) {
+  __shared__ __align__(16) uint16_t tb[kHeadRows][kHbStride];  // dM of the tile, BF16, for the A operand
+  const int lane = threadIdx.x, c = lane & 31, g = lane >> 5;
+  const int i = (int)blockIdx.x * kHeadRows + c;  // A-operand row c
+  const bool live = i < n;
+  const float *prm = m.params;
+  const float os = prm[kOffNorm + 5];
+  float d0 = 0.0f, d1 = 0.0f, err = 0.0f, ref = 0.0f;
+  if (live) {
     const float2 t = m.target[i];
     const float t0 = t.x / os, t1 = t.y / os;
     const float e0 = m.out[2 * (size_t)i] - t0;
     const float e1 = m.out[2 * (size_t)i + 1] - t1;
-    err = e0 * e0 + e1 * e1;
-    ref = t0 * t0 + t1 * t1;
-    const float d0 = 2.0f * invN * e0, d1 = 2.0f * invN * e1;
-    m.dO[2 * (size_t)i] = d0;
-    m.dO[2 * (size_t)i + 1] = d1;
-    float *dG = m.dG + (size_t)i * kHid, *dM = m.dM + (size_t)i * kHid, *dH = m.dH + (size_t)i * kHid;
-    float vin[kHid], acc[kHid];
-#pragma clang loop unroll(full)
-    for (int k = 0; k < kHid; ++k) {
-      const float v = __builtin_fmaf(prm[kOffW4 + k], d0, prm[kOffW4 + kHid + k] * d1);
-      vin[k] = G[k] > 0.0f ? v : 0.0f;
-      dG[k] = vin[k];
-      acc[k] = 0.0f;
+    d0 = 2.0f * invN * e0;
+    d1 = 2.0f * invN * e1;
+    if (g == 0) {
+      err = e0 * e0 + e1 * e1;
+      ref = t0 * t0 + t1 * t1;
+      m.dO[2 * (size_t)i] = d0;
+      m.dO[2 * (size_t)i + 1] = d1;
     }
-    // dM = W3^T dG, then dH = W2^T dM: row r of W adds W[r][:] * d[r]
-#pragma clang loop unroll(full)
-    for (int r = 0; r < kHid; ++r) {
-#pragma clang loop unroll(full)
-      for (int k = 0; k < kHid; ++k) acc[k] = __builtin_fmaf(prm[kOffW3 + r * kHid + k], vin[r], acc[k]);
-    }
-#pragma clang loop unroll(full)
-    for (int k = 0; k < kHid; ++k) {
-      dM[k] = acc[k];
-      vin[k] = acc[k];
-      acc[k] = 0.0f;
-    }
-#pragma clang loop unroll(full)
-    for (int r = 0; r < kHid; ++r) {
-#pragma clang loop unroll(full)
-      for (int k = 0; k < kHid; ++k) acc[k] = __builtin_fmaf(prm[kOffW2 + r * kHid + k], vin[r], acc[k]);
-    }
-#pragma clang loop unroll(full)
-    for (int k = 0; k < kHid; ++k) dH[k] = acc[k];
   }
-  // squared error and squared target, summed per workgroup
+  // dG for this lane's K indices k = 16s + 8g + u of row c
+  bf16x8 ag[4];
+  {
+    const float *G = m.act + (size_t)(live ? i : 0) * kActStride + 2 * kHid;
+    float *dG = m.dG + (size_t)(live ? i : 0) * kHid;
+#pragma unroll
+    for (int st = 0; st < 4; ++st) {
+      float v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = 16 * st + 8 * g + u;
+        const float x = __builtin_fmaf(prm[kOffW4 + k], d0, prm[kOffW4 + kHid + k] * d1);
+        v[u] = (live && G[k] > 0.0f) ? x : 0.0f;
+        if (live) dG[k] = v[u];
+      }
+      ag[st] = bf8(pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7]));
+    }
+  }
+  // dM = dG W3: B[k][n] = W3[k][n] = W3t[n][k]
+  floatx16 mlo = zero16(), mhi = zero16();
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    mlo = mfma16(ag[st], load_wb16(m.w3t, kHid, 0, 16 * st), mlo);
+    mhi = mfma16(ag[st], load_wb16(m.w3t, kHid, 32, 16 * st), mhi);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = c_row(r, g);
+    const int pi = (int)blockIdx.x * kHeadRows + row;
+    if (pi < n) {
+      m.dM[(size_t)pi * kHid + c] = mlo[r];
+      m.dM[(size_t)pi * kHid + 32 + c] = mhi[r];
+    }
+    tb[row][c] = f2bf(mlo[r]);
+    tb[row][32 + c] = f2bf(mhi[r]);
+  }
+  __syncthreads();
+  // dH = dM W2
+  floatx16 hlo = zero16(), hhi = zero16();
+#pragma unroll
+  for (int st = 0; st < 4; ++st) {
+    const bf16x8 a = lds_bf8(&tb[c][16 * st + 8 * g]);
+    hlo = mfma16(a, load_wb16(m.w2t, kHid, 0, 16 * st), hlo);
+    hhi = mfma16(a, load_wb16(m.w2t, kHid, 32, 16 * st), hhi);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int pi = (int)blockIdx.x * kHeadRows + c_row(r, g);
+    if (pi < n) {
+      m.dH[(size_t)pi * kHid + c] = hlo[r];
+      m.dH[(size_t)pi * kHid + 32 + c] = hhi[r];
+    }
+  }
+  // squared error and squared target of the wave's particles
   for (int off = 32; off > 0; off >>= 1) {
     err += __shfl_xor(err, off, 64);
     ref += __shfl_xor(ref, off, 64);
   }
-  if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = err;
-    red[1][threadIdx.x >> 6] = ref;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float s0 = 0.0f, s1 = 0.0f;
-    for (int w = 0; w < kBlock / 64; ++w) s0 += red[0][w], s1 += red[1][w];
-    m.lossPart[2 * blockIdx.x] = s0;
-    m.lossPart[2 * blockIdx.x + 1] = s1;
+  if (lane == 0) {
+    m.lossPart[2 * blockIdx.x] = err;
+    m.lossPart[2 * blockIdx.x + 1] = ref;
   }
 }
 
@@ -526,41 +612,61 @@ __global__ __launch_bounds__(kBlock) void k_loss_final(const float *part, int nb
   }
 }
 
-// part[blockIdx][a * nb + b] = sum over the workgroup's rows of A[i][a] B[i][b];
-// part[blockIdx][na * nb + a] = sum of A[i][a] * scale_i (scale 1 if null).
-// Workgroups stride over rows in tiles of 32.
-__global__ __launch_bounds__(kBlock) void k_outer_partial(int n, const float *A, int lda, int na, const float *B,
-                                                          int ldb, int nb, const float *scale, float *part) {
-  __shared__ float sa[32][kHid], sb[32][kHid], ss[32];
-  const int t = threadIdx.x;
-  const int a0 = 4 * (t / 16), b0 = 4 * (t % 16);
-  float acc[4][4] = {};
-  float accb[4] = {};
-  for (int base = (int)blockIdx.x * 32; base < n; base += (int)gridDim.x * 32) {
-    for (int e = t; e < 32 * kHid; e += kBlock) {
-      const int rr = e / kHid, k = e % kHid, i = base + rr;
-      sa[rr][k] = (i < n && k < na) ? A[(size_t)i * lda + k] : 0.0f;
-      sb[rr][k] = (i < n && k < nb) ? B[(size_t)i * ldb + k] : 0.0f;
-      if (k == 0) ss[rr] = (i < n) ? (scale ? scale[i] : 1.0f) : 0.0f;
-    }
-    __syncthreads();
-    for (int rr = 0; rr < 32; ++rr) {
+// Weight-gradient partials on the matrix core: for the particles of this
+// wave's K = 16 blocks, part[wave][a * 64 + b] = sum_i bf16(A[i][a]) bf16(B[i][b])
+// (FP32 accumulation) and part[wave][na * 64 + a] = sum_i A[i][a] scale_i
+// (FP32; scale 1 if null), for na = 2 or 64 rows of A and 64 columns of B.
+// A K = 16 step takes particles k0 + 8g + u (u = 0..7) in lane group g: the A
+// operand holds A[i][a] of row a = c (+32) and the B operand B[i][b] of
+// column b = c (+32), 32-lane coalesced row reads.
+constexpr int kOuterWaves = 2048;
+__global__ __launch_bounds__(64) void k_outer_partial(int n, const float *A, int lda, int na, const float *B, int ldb,
+                                                      const float *scale, float *part) {
+  const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const int aHalves = na > 32 ? 2 : 1;
+  floatx16 acc[2][2];
 #pragma unroll
-      for (int x = 0; x < 4; ++x) {
-        const float av = sa[rr][a0 + x];
-        accb[x] = __builtin_fmaf(av, ss[rr], accb[x]);
+  for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 4; ++y) acc[x][y] = __builtin_fmaf(av, sb[rr][b0 + y], acc[x][y]);
+    for (int y = 0; y < 2; ++y) acc[x][y] = zero16();
+  float bias[2] = {0.0f, 0.0f};
+  for (int k0 = (int)blockIdx.x * 16; k0 < n; k0 += (int)gridDim.x * 16) {
+    bf16x8 ao[2], bo[2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      float va[8], vb[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = k0 + 8 * g + u;
+        const bool in = i < n;
+        const int a = 32 * x + c;
+        va[u] = (in && a < na) ? A[(size_t)i * lda + a] : 0.0f;
+        vb[u] = in ? B[(size_t)i * ldb + 32 * x + c] : 0.0f;
+        if (in) bias[x] = __builtin_fmaf(va[u], scale ? scale[i] : 1.0f, bias[x]);
       }
+      ao[x] = bf8(pack2(va[0], va[1]), pack2(va[2], va[3]), pack2(va[4], va[5]), pack2(va[6], va[7]));
+      bo[x] = bf8(pack2(vb[0], vb[1]), pack2(vb[2], vb[3]), pack2(vb[4], vb[5]), pack2(vb[6], vb[7]));
     }
-    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      if (x >= aHalves) break;
+#pragma unroll
+      for (int y = 0; y < 2; ++y) acc[x][y] = mfma16(ao[x], bo[y], acc[x][y]);
+    }
   }
-  float *out = part + (size_t)blockIdx.x * (size_t)(na * nb + na);
-  for (int x = 0; x < 4; ++x) {
-    if (a0 + x >= na) continue;
-    for (int y = 0; y < 4; ++y)
-      if (b0 + y < nb) out[(a0 + x) * nb + b0 + y] = acc[x][y];
-    if (b0 == 0) out[na * nb + a0 + x] = accb[x];
+  float *out = part + (size_t)blockIdx.x * (size_t)(na * kHid + na);
+#pragma unroll
+  for (int x = 0; x < 2; ++x) {
+    if (x >= aHalves) break;
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int a = 32 * x + c_row(r, g);
+        if (a < na) out[a * kHid + 32 * y + c] = acc[x][y][r];
+      }
+    const float tot = bias[x] + __shfl_xor(bias[x], 32, 64);
+    if (g == 0 && 32 * x + c < na) out[na * kHid + 32 * x + c] = tot;
   }
 }
 
@@ -586,12 +692,12 @@ __global__ __launch_bounds__(kBlock) void k_scatter_grad(const float *sum, int n
 }
 
 // dW1 = sum_i sum_j (dH_i . [z_ij > 0]) f_ij^T, with the pair walk of the
-// forward pass.  For row tile t the pair layer's accumulator (C layout: lane =
-// hidden unit, register 4k+u = tile row 8k+4g+u), masked and scaled by the
-// rows' dH, is the A operand (hidden x 8 rows) of the k-th 8-row block; the
-// rows' features, staged in LDS, are the B operand (8 rows x 32 features, 16
-// used).  Each wave accumulates over its particles and writes one 64 x 16
-// partial.
+// forward pass.  For row tile t the pair layer's pre-activations (C layout:
+// lane = hidden unit, register r = tile row c_row(r, g)), masked and scaled by
+// the rows' dH, are the A operand (hidden x 16 rows) of K step s (registers
+// 8s .. 8s + 7); the same rows' features, staged in LDS, are the B operand
+// (16 rows x 32 features, 16 used).  Each wave accumulates over its particles
+// and writes one 64 x 16 partial.
 template <bool MULTI>
 __global__ __launch_bounds__(64) void k_sur_back_pairs(Buffers b, DevParams p, DevScene s0, int n, SurrogateBufs m) {
   __shared__ uint32_t fw[kWaveRows][kFeat / 2];
@@ -636,28 +742,35 @@ __global__ __launch_bounds__(64) void k_sur_back_pairs(Buffers b, DevParams p, D
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
         floatx16 zlo, zhi;
-        pair_layer(t ? a.t1s0 : a.t0s0, t ? a.t1s1 : a.t0s1, w1, zlo, zhi);
+        pair_layer(t ? a.t1 : a.t0, w1, zlo, zhi);
         const floatx16 &dlo = t ? d10 : d00;
         const floatx16 &dhi = t ? d11 : d01;
+        // K step st: K index 8g + u is tile row c_row(8st + u, g), this lane's
+        // register 8st + u (A: masked dH of hidden c; B: that row's feature c)
 #pragma unroll
-        for (int blk = 0; blk < 4; ++blk) {
-          bf16x4 bf;
+        for (int st = 0; st < 2; ++st) {
+          uint32_t bw[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const uint32_t wd = fw[32 * t + 8 * blk + 4 * g + u][(c & 15) >> 1];
-            const uint16_t hv = (c & 1) ? (uint16_t)(wd >> 16) : (uint16_t)(wd & 0xffffu);
-            bf[u] = (short)(c < kFeat ? hv : 0);
+          for (int q = 0; q < 4; ++q) {
+            uint32_t hv[2];
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+              const uint32_t wd = fw[32 * t + c_row(8 * st + 2 * q + e, g)][(c & 15) >> 1];
+              hv[e] = c < kFeat ? ((c & 1) ? (wd >> 16) : (wd & 0xffffu)) : 0u;
+            }
+            bw[q] = hv[0] | (hv[1] << 16);
           }
-          float ml[4], mh[4];
+          float ml[8], mh[8];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            ml[u] = zlo[4 * blk + u] > 0.0f ? dlo[4 * blk + u] : 0.0f;
-            mh[u] = zhi[4 * blk + u] > 0.0f ? dhi[4 * blk + u] : 0.0f;
+          for (int u = 0; u < 8; ++u) {
+            ml[u] = zlo[8 * st + u] > 0.0f ? dlo[8 * st + u] : 0.0f;
+            mh[u] = zhi[8 * st + u] > 0.0f ? dhi[8 * st + u] : 0.0f;
           }
-          const bf16x4 alo = words_to_bf4(pack2(ml[0], ml[1]), pack2(ml[2], ml[3]));
-          const bf16x4 ahi = words_to_bf4(pack2(mh[0], mh[1]), pack2(mh[2], mh[3]));
-          glo = mfma(alo, bf, glo);
-          ghi = mfma(ahi, bf, ghi);
+          const bf16x8 bf = bf8(bw[0], bw[1], bw[2], bw[3]);
+          glo = mfma16(bf8(pack2(ml[0], ml[1]), pack2(ml[2], ml[3]), pack2(ml[4], ml[5]), pack2(ml[6], ml[7])), bf,
+                       glo);
+          ghi = mfma16(bf8(pack2(mh[0], mh[1]), pack2(mh[2], mh[3]), pack2(mh[4], mh[5]), pack2(mh[6], mh[7])), bf,
+                       ghi);
         }
       }
       __syncthreads();
@@ -674,13 +787,18 @@ __global__ __launch_bounds__(64) void k_sur_back_pairs(Buffers b, DevParams p, D
   }
 }
 
+// BF16 copies of W1..W3 and the transposes of W2, W3 (w2t[n][k] = W2[k][n]) for
+// the backward pass's B operands.
 __global__ __launch_bounds__(kBlock) void k_bf16_weights(const float *params, uint16_t *w1b, uint16_t *w2b,
-                                                         uint16_t *w3b) {
+                                                         uint16_t *w3b, uint16_t *w2t, uint16_t *w3t) {
   const int t = (int)(blockIdx.x * kBlock + threadIdx.x);
   if (t < kHid * kFeat) w1b[t] = f2bf(params[kOffW1 + t]);
   if (t < kHid * kHid) {
     w2b[t] = f2bf(params[kOffW2 + t]);
     w3b[t] = f2bf(params[kOffW3 + t]);
+    const int r = t / kHid, k = t % kHid;
+    w2t[k * kHid + r] = f2bf(params[kOffW2 + t]);
+    w3t[k * kHid + r] = f2bf(params[kOffW3 + t]);
   }
 }
 
@@ -827,12 +945,28 @@ __global__ __launch_bounds__(64) void k_mfma_probe(const float *A, const float *
   for (int k = 0; k < 16; ++k) C[c_row(k, g) * 32 + c] = r[k];
 }
 
+// K = 16 layout self-test: C (32 x 32) = A (32 x 16) B (16 x 32).
+__global__ __launch_bounds__(64) void k_mfma16_probe(const float *A, const float *B, float *C) {
+  const int lane = threadIdx.x & 63, c = lane & 31, g = lane >> 5;
+  float a[8], bb[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    a[t] = A[c * 16 + 8 * g + t];
+    bb[t] = B[(8 * g + t) * 32 + c];
+  }
+  const floatx16 r = mfma16(bf8(pack2(a[0], a[1]), pack2(a[2], a[3]), pack2(a[4], a[5]), pack2(a[6], a[7])),
+                            bf8(pack2(bb[0], bb[1]), pack2(bb[2], bb[3]), pack2(bb[4], bb[5]), pack2(bb[6], bb[7])),
+                            zero16());
+#pragma unroll
+  for (int k = 0; k < 16; ++k) C[c_row(k, g) * 32 + c] = r[k];
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 static unsigned waves_for(int n) { return (unsigned)((n + kWaveRows - 1) / kWaveRows); }
 static unsigned back_pair_waves(int n) { return std::min(waves_for(n), 4096u); }
-static unsigned outer_blocks(int n) { return (unsigned)std::min((n + kOuterRows - 1) / kOuterRows, 1024); }
+static unsigned outer_blocks(int n) { return (unsigned)std::max(1, std::min((n + 15) / 16, kOuterWaves)); }
 
 size_t surrogate_part_floats(int n) {
   const size_t a = (size_t)back_pair_waves(n) * kHid * kFeat;
@@ -840,7 +974,7 @@ size_t surrogate_part_floats(int n) {
   return std::max<size_t>(std::max(a, o), 1) + (kHid * kHid + kHid);  // + the reduced block
 }
 
-int surrogate_head_blocks(int n) { return std::max(1, (n + kBlock - 1) / kBlock); }
+int surrogate_head_blocks(int n) { return std::max(1, (n + kHeadRows - 1) / kHeadRows); }
 int surrogate_forward_waves(int n) { return std::max(1, (int)waves_for(n)); }
 
 void launch_surrogate_forward(const Buffers &b, const DevParams &p, const DevScene &s0, int n,
@@ -859,7 +993,7 @@ static void outer_grad(int n, const float *A, int lda, int na, const float *B, i
   const unsigned blocks = outer_blocks(n);
   const int width = na * nb + na;
   float *sum = part + (size_t)blocks * width;
-  hipLaunchKernelGGL(k_outer_partial, dim3(blocks), dim3(kBlock), 0, s, n, A, lda, na, B, ldb, nb, scale, part);
+  hipLaunchKernelGGL(k_outer_partial, dim3(blocks), dim3(64), 0, s, n, A, lda, na, B, ldb, scale, part);
   hipLaunchKernelGGL(k_sum_parts, dim3((width + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (const float *)part,
                      (int)blocks, width, sum);
   hipLaunchKernelGGL(k_scatter_grad, dim3((width + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (const float *)sum, na,
@@ -871,7 +1005,7 @@ void launch_surrogate_backward(const Buffers &b, const DevParams &p, const DevSc
   if (n <= 0) return;
   const int hb = surrogate_head_blocks(n);
   const float invN = 1.0f / (float)n;
-  hipLaunchKernelGGL(k_sur_back_head, dim3(hb), dim3(kBlock), 0, s, n, m, invN);
+  hipLaunchKernelGGL(k_sur_back_head, dim3(hb), dim3(64), 0, s, n, m, invN);
   hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(kBlock), 0, s, (const float *)m.lossPart, hb, invN,
                      m.loss + 2 * (size_t)m.lossIndex);
   // dW4 = sum dO G^T, db4 = sum dO
@@ -891,9 +1025,10 @@ void launch_surrogate_backward(const Buffers &b, const DevParams &p, const DevSc
                      (const float *)m.part, (int)waves, kHid * kFeat, m.grad + kOffW1);
 }
 
-void launch_bf16_weights(const float *params, uint16_t *w1b, uint16_t *w2b, uint16_t *w3b, hipStream_t s) {
+void launch_bf16_weights(const float *params, uint16_t *w1b, uint16_t *w2b, uint16_t *w3b, uint16_t *w2t,
+                         uint16_t *w3t, hipStream_t s) {
   hipLaunchKernelGGL(k_bf16_weights, dim3((kHid * kHid + kBlock - 1) / kBlock), dim3(kBlock), 0, s, params, w1b, w2b,
-                     w3b);
+                     w3b, w2t, w3t);
 }
 
 void launch_adam(float *params, const float *grad, float *mo, float *ve, int count, float lr, float b1, float b2,
@@ -919,6 +1054,10 @@ void launch_density_error_scenes(const float2 *a, const float2 *bb, const uint32
 
 void launch_mfma_probe(const float *a, const float *b, float *c, hipStream_t s) {
   hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, s, a, b, c);
+}
+
+void launch_mfma16_probe(const float *a, const float *b, float *c, hipStream_t s) {
+  hipLaunchKernelGGL(k_mfma16_probe, dim3(1), dim3(64), 0, s, a, b, c);
 }
 
 }  // namespace nbs

@@ -31,7 +31,7 @@
 namespace nbs {
 
 constexpr int kHid = 64;
-constexpr int kFeat = 16;  // features per pair (MFMA K = 2 x 8)
+constexpr int kFeat = 16;  // features per pair (one MFMA K step)
 
 // Offsets into the flat FP32 parameter vector.
 constexpr int kOffW1 = 0;                       // 64 x 16 (column 8 acts as the bias)
@@ -55,6 +55,7 @@ struct SurrogateBufs {
   const uint16_t *w1b;       // 64 x 16 BF16
   const uint16_t *w2b;       // 64 x 64 BF16
   const uint16_t *w3b;       // 64 x 64 BF16
+  const uint16_t *w2t, *w3t; // transposes of W2, W3 (BF16), for the backward pass
   float *out;                // N x 2 network output o (may be null)
   float *act;                // N x 192: H, M, G (training)
   float *cnt;                // N: pairs per particle (training)
@@ -85,7 +86,8 @@ int surrogate_forward_waves(int n);
 // Sizes of the partial-sum buffer (floats) and of lossPart (pairs) for n particles.
 size_t surrogate_part_floats(int n);
 int surrogate_head_blocks(int n);
-void launch_bf16_weights(const float *params, uint16_t *w1b, uint16_t *w2b, uint16_t *w3b, hipStream_t stream);
+void launch_bf16_weights(const float *params, uint16_t *w1b, uint16_t *w2b, uint16_t *w3b, uint16_t *w2t,
+                         uint16_t *w3t, hipStream_t stream);
 void launch_adam(float *params, const float *grad, float *m, float *v, int count, float lr, float b1, float b2,
                  float eps, int t, hipStream_t stream);
 // out[0] = sum |a.x - b.x|, out[1] = sum b.x over n (FP64), partials sized
@@ -98,5 +100,7 @@ void launch_density_error_scenes(const float2 *a, const float2 *b, const uint32_
                                  int nScenes, int n, double *out, hipStream_t stream);
 // C (32 x 32) = A (32 x 8) B (8 x 32), row-major FP32 inputs rounded to BF16.
 void launch_mfma_probe(const float *a, const float *b, float *c, hipStream_t stream);
+// C (32 x 32) = A (32 x 16) B (16 x 32), one v_mfma_f32_32x32x16_bf16.
+void launch_mfma16_probe(const float *a, const float *b, float *c, hipStream_t stream);
 
 }  // namespace nbs

@@ -379,6 +379,15 @@ def selftest_mfma(a, b) -> np.ndarray:
     return c
 
 
+def selftest_mfma16(a, b) -> np.ndarray:
+    """C = A (32 x 16) B (16 x 32) with BF16-rounded operands by one K = 16 MFMA."""
+    a = np.ascontiguousarray(a, np.float32).reshape(32, 16)
+    b = np.ascontiguousarray(b, np.float32).reshape(16, 32)
+    c = np.zeros((32, 32), np.float32)
+    call("nbs_selftest_mfma16", _ptr(a), _ptr(b), _ptr(c))
+    return c
+
+
 def bench_copy(nbytes: int = 1 << 30, reps: int = 20) -> float:
     """Device copy bandwidth (read + write GB/s) of the library's copy kernel."""
     g = ctypes.c_double()

@@ -21,9 +21,14 @@ near density per particle, scene per particle) and the parameter vector:
   loss      L = (1/N) sum_i |o_i - target_i / os|^2.
 
 With ``bf16=True`` the operands the GPU feeds to its BF16 matrix cores are
-rounded the same way (features, W1..W3, H, M; the W1 gradient's masked dH),
-everything else is float64; with ``bf16=False`` the whole network is float64,
-which is what the finite-difference checks of the gradient use.
+rounded the same way, everything else is float64:
+  forward   features, W1..W3, each pair's relu(z) (the GPU sums the BF16
+            values on the matrix core), H, M;
+  backward  dG and dM (the operands of dM = dG W3, dH = dM W2 and of the W3,
+            W2 gradients), W2, W3, dO and G in the W4 gradient, and the W1
+            gradient's masked dH; the bias gradients sum the unrounded values.
+With ``bf16=False`` the whole network is float64, which is what the
+finite-difference checks of the gradient use.
 """
 from __future__ import annotations
 
@@ -133,8 +138,11 @@ def forward(params, frame, h, dt, bf16_ops=True):
     W2 = rnd(w["W2"]).astype(np.float64)
     W3 = rnd(w["W3"]).astype(np.float64)
     Z = Fb @ W1.T                                        # pairs x 64
+    R = np.maximum(Z, 0.0)
+    if bf16_ops:
+        R = bf16(R.astype(np.float32)).astype(np.float64)
     H = np.zeros((n, HID))
-    np.add.at(H, pi, np.maximum(Z, 0.0))
+    np.add.at(H, pi, R)
     cnt = np.bincount(pi, minlength=n).astype(np.float64)
     Hin = rnd(H.astype(np.float32)).astype(np.float64) if bf16_ops else H
     M = Hin @ W2.T + cnt[:, None] * w["b2"].astype(np.float64)
@@ -159,9 +167,12 @@ def forward_rows(params, frame, h, dt, rows, bf16_ops=True):
     F = features(frame, pi, pj, h, dt, w["norm"])
     rnd = bf16 if bf16_ops else (lambda a: np.asarray(a, np.float64))
     Z = rnd(F).astype(np.float64) @ rnd(w["W1"]).astype(np.float64).T
+    R = np.maximum(Z, 0.0)
+    if bf16_ops:
+        R = bf16(R.astype(np.float32)).astype(np.float64)
     li = local[pi]
     H = np.zeros((rows.size, HID))
-    np.add.at(H, li, np.maximum(Z, 0.0))
+    np.add.at(H, li, R)
     cnt = np.bincount(li, minlength=rows.size).astype(np.float64)
     Hin = rnd(H.astype(np.float32)).astype(np.float64) if bf16_ops else H
     M = Hin @ rnd(w["W2"]).astype(np.float64).T + cnt[:, None] * w["b2"].astype(np.float64)
@@ -171,8 +182,8 @@ def forward_rows(params, frame, h, dt, rows, bf16_ops=True):
 
 
 def loss_and_grad(params, frame, h, dt, bf16_ops=True):
-    """Loss and the gradient of the NUM_WEIGHTS trained parameters (float64).
-    The backward pass uses the float32 master weights, as the GPU's does."""
+    """Loss and the gradient of the NUM_WEIGHTS trained parameters (float64),
+    with the GPU's BF16 operand roundings when bf16_ops (module docstring)."""
     w = unpack(params)
     o, c = forward(params, frame, h, dt, bf16_ops)
     n = o.shape[0]
@@ -181,18 +192,22 @@ def loss_and_grad(params, frame, h, dt, bf16_ops=True):
     e = o - t
     L = float((e * e).sum() / n)
     dO = 2.0 * e / n
+    rnd = (lambda a: bf16(np.asarray(a, np.float32)).astype(np.float64)) if bf16_ops else \
+        (lambda a: np.asarray(a, np.float64))
     W4 = w["W4"].astype(np.float64)
-    W3 = w["W3"].astype(np.float64)
-    W2 = w["W2"].astype(np.float64)
+    W3 = rnd(w["W3"])
+    W2 = rnd(w["W2"])
     dGpre = (dO @ W4) * (c["G"] > 0.0)
-    dM = dGpre @ W3
-    dH = dM @ W2
+    dGr = rnd(dGpre)
+    dM = dGr @ W3
+    dMr = rnd(dM)
+    dH = dMr @ W2
     g = np.zeros(NUM_WEIGHTS)
-    g[OFF_W4:OFF_B4] = (dO.T @ c["G"]).reshape(-1)
+    g[OFF_W4:OFF_B4] = (rnd(dO).T @ rnd(c["G"])).reshape(-1)
     g[OFF_B4:NUM_WEIGHTS] = dO.sum(0)
-    g[OFF_W3:OFF_B3] = (dGpre.T @ c["Min"]).reshape(-1)
+    g[OFF_W3:OFF_B3] = (dGr.T @ c["Min"]).reshape(-1)
     g[OFF_B3:OFF_W4] = dGpre.sum(0)
-    g[OFF_W2:OFF_B2] = (dM.T @ c["Hin"]).reshape(-1)
+    g[OFF_W2:OFF_B2] = (dMr.T @ c["Hin"]).reshape(-1)
     g[OFF_B2:OFF_W3] = (dM * c["cnt"][:, None]).sum(0)
     A = dH[c["pi"]] * (c["Z"] > 0.0)                     # pairs x 64
     if c["bf16"]:

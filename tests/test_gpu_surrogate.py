@@ -1,8 +1,8 @@
 """The learned relaxation on the GPU against its CPU oracle, and what training
 buys.
 
-* the MFMA operand layout the kernels assume (one v_mfma_f32_32x32x8_bf16
-  against a host product);
+* the MFMA operand layouts the kernels assume (one v_mfma_f32_32x32x8_bf16 and
+  one v_mfma_f32_32x32x16_bf16 against host products);
 * the forward output, the loss and every block of the weight gradient of one
   frame against oracle/surrogate_oracle.py, which rounds the same operands
   to BF16 and computes the rest in float64 (single scene and batched scenes);
@@ -10,10 +10,10 @@ buys.
   below that of the untrained network and of dropping the relaxation;
 * a density check that always fails reproduces the plain method bit for bit.
 
-Tolerances: the GPU accumulates in FP32 in its own order and rounds H and M
-to BF16 from FP32 sums, the oracle from float64 sums; an element whose two
-sums straddle a BF16 rounding boundary moves by one BF16 ulp (2^-8
-relative).  Measured relative L2 errors are about 1e-6 (outputs, loss and
+Tolerances: the GPU accumulates in FP32 in its own order and rounds each
+pair's relu(z), H, M, dG and dM to BF16 from FP32 sums, the oracle from
+float64 sums; an element whose two sums straddle a BF16 rounding boundary
+moves by one BF16 ulp (2^-8 relative).  Measured relative L2 errors are about 1e-6 (outputs, loss and
 every gradient block); the bound is 1e-3.
 """
 from __future__ import annotations
@@ -46,6 +46,19 @@ def test_mfma_operand_layout():
     b = rng.normal(size=(8, 32)).astype(np.float32)
     want = so().bf16(a).astype(np.float64) @ so().bf16(b).astype(np.float64)
     np.testing.assert_allclose(selftest_mfma(a, b), want, rtol=1e-5, atol=1e-5)
+
+
+def test_mfma16_operand_layout():
+    """The K = 16 form the network's kernels use: A[c][8g + t], B[8g + t][c]."""
+    from nbodysimulation_experiment_amd.sim import selftest_mfma16
+    rng = np.random.default_rng(1)
+    a = rng.integers(-8, 8, (32, 16)).astype(np.float32)
+    b = rng.integers(-8, 8, (16, 32)).astype(np.float32)
+    assert np.array_equal(selftest_mfma16(a, b), a @ b)
+    a = rng.normal(size=(32, 16)).astype(np.float32)
+    b = rng.normal(size=(16, 32)).astype(np.float32)
+    want = so().bf16(a).astype(np.float64) @ so().bf16(b).astype(np.float64)
+    np.testing.assert_allclose(selftest_mfma16(a, b), want, rtol=1e-5, atol=1e-5)
 
 
 def rel(a, b):
