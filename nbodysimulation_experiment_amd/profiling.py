@@ -34,6 +34,7 @@ SHORT = [
     ("k_count_scan_small", "count_scan_small"), ("k_scan_reduce", "scan_reduce"),
     ("k_scan_small", "scan_small"),
     ("k_scan_final", "scan_final"), ("k_scatter_grad", "scatter_grad"), ("k_scatter", "scatter"), ("k_rank_reorder", "rank_reorder"),
+    ("k_place", "place"),
     ("k_density", "density"), ("k_delta", "delta"), ("k_copy", "copy"), ("k_sur_forward", "sur_forward"),
     ("k_sur_back_pairs", "sur_back_pairs"), ("k_sur_back_head", "sur_back_head"),
 ]
@@ -43,8 +44,8 @@ SHORT = [
 # the one-lane-per-particle and in the wide form.
 GROUPS = {
     "viscosity_predict": (("viscosity_predict",), ("viscosity_predict_wide", "count_fill"), ("viscosity_predict_wide",)),
-    "grid": (("scan_reduce", "scan_final", "scatter", "rank_reorder"), ("scan_small", "scatter", "rank_reorder"),
-             ("count_scan_small", "scatter", "rank_reorder")),
+    "grid": (("scan_reduce", "scan_final", "place"), ("scan_reduce", "scan_final", "scatter", "rank_reorder"),
+             ("scan_small", "scatter", "rank_reorder"), ("count_scan_small", "scatter", "rank_reorder")),
     "density": (("density",), ("density_wide",)),
     "delta": (("delta",), ("delta_wide",)),
     "surrogate_forward": (("sur_forward",),),
