@@ -34,13 +34,14 @@ typedef short bf16x4 __attribute__((__vector_size__(4 * sizeof(short))));
 typedef float floatx16 __attribute__((__vector_size__(16 * sizeof(float))));
 
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
-// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32).
+// FP32 -> BF16, round to nearest even: one v_cvt_pk_bf16_f32 for both values
+// (converting the elements one at a time costs two conversions and a permute
+// when the inputs are MFMA results).
 __device__ __forceinline__ uint32_t pack2(float lo, float hi) {
-  bf16x2 v;
-  v[0] = (__bf16)lo;
-  v[1] = (__bf16)hi;
-  return __builtin_bit_cast(uint32_t, v);
+  const f32x2 f = {lo, hi};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2));
 }
 __device__ __forceinline__ uint16_t f2bf(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
@@ -276,12 +277,33 @@ __device__ __forceinline__ bool next_pair(Cursor &cu, const Buffers &b, const De
     // m = 2 * m + (d2 < h2), last candidate first, so candidate k ends at bit
     // k: the compare's lane mask is the add's carry-in
     uint32_t m = 0;
-    for (uint32_t u = cnt; u > 0; --u) {
-      const float2 x = src.x[cu.j + u - 1 + cu.tb];
-      const float dx = x.x - q.xi.x, dy = x.y - q.xi.y;
-      const float d2 = dx * dx + dy * dy;
+    // the chunk's first index in the source: j + tb wraps modulo 2^32 (tb is
+    // off - rowStart on the LDS tile), so it is formed in 32 bits first
+    const float2 *xs = src.x + (uint32_t)(cu.j + cu.tb);
+    auto push = [&](float d2) {
       asm("v_cmp_gt_f32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc" : "+v"(m) : "s"(p.h2), "v"(d2) : "vcc");
+    };
+    auto dist2 = [&](uint32_t k) {
+      const float2 x = xs[k];
+      const float dx = x.x - q.xi.x, dy = x.y - q.xi.y;
+      return dx * dx + dy * dy;
+    };
+    uint32_t u = cnt;
+    for (; u >= 8u; u -= 8u) {  // eight position loads in flight
+      float d2[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) d2[k] = dist2(u - 1 - k);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) push(d2[k]);
     }
+    for (; u >= 4u; u -= 4u) {
+      const float a0 = dist2(u - 1), a1 = dist2(u - 2), a2 = dist2(u - 3), a3 = dist2(u - 4);
+      push(a0);
+      push(a1);
+      push(a2);
+      push(a3);
+    }
+    for (; u > 0u; --u) push(dist2(u - 1));
     const uint32_t self = (uint32_t)i - cu.j;  // the particle itself is no pair
     if (self < cnt) m &= ~(1u << self);
     cu.mbase = cu.j;
@@ -592,14 +614,25 @@ __global__ __launch_bounds__(64) void k_sur_back_head(int n, SurrogateBufs m, fl
 }
 
 // loss[0] = mean squared error, loss[1] = sum |e|^2 / sum |target/os|^2.
-__global__ __launch_bounds__(kBlock) void k_loss_final(const float *part, int nb, float invN, float *loss) {
-  __shared__ float red[2][kBlock];
-  float s0 = 0.0f, s1 = 0.0f;
-  for (int k = threadIdx.x; k < nb; k += kBlock) s0 += part[2 * k], s1 += part[2 * k + 1];
-  red[0][threadIdx.x] = s0;
-  red[1][threadIdx.x] = s1;
+constexpr int kLossBlock = 1024;
+__global__ __launch_bounds__(kLossBlock) void k_loss_final(const float *part, int nb, float invN, float *loss) {
+  __shared__ float red[2][kLossBlock];
+  // four partials in flight per thread, summed in a fixed order
+  const float2 *pp = reinterpret_cast<const float2 *>(part);
+  float2 acc[4] = {{0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f}, {0.0f, 0.0f}};
+  int k = (int)threadIdx.x;
+  for (; k + 3 * kLossBlock < nb; k += 4 * kLossBlock) {
+    float2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = pp[k + u * kLossBlock];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u].x += v[u].x, acc[u].y += v[u].y;
+  }
+  for (; k < nb; k += kLossBlock) acc[0].x += pp[k].x, acc[0].y += pp[k].y;
+  red[0][threadIdx.x] = (acc[0].x + acc[1].x) + (acc[2].x + acc[3].x);
+  red[1][threadIdx.x] = (acc[0].y + acc[1].y) + (acc[2].y + acc[3].y);
   __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
+  for (int w = kLossBlock / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
       red[0][threadIdx.x] += red[0][threadIdx.x + w];
       red[1][threadIdx.x] += red[1][threadIdx.x + w];
@@ -1006,7 +1039,7 @@ void launch_surrogate_backward(const Buffers &b, const DevParams &p, const DevSc
   const int hb = surrogate_head_blocks(n);
   const float invN = 1.0f / (float)n;
   hipLaunchKernelGGL(k_sur_back_head, dim3(hb), dim3(64), 0, s, n, m, invN);
-  hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(kBlock), 0, s, (const float *)m.lossPart, hb, invN,
+  hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(kLossBlock), 0, s, (const float *)m.lossPart, hb, invN,
                      m.loss + 2 * (size_t)m.lossIndex);
   // dW4 = sum dO G^T, db4 = sum dO
   outer_grad(n, m.dO, 2, 2, m.act + 2 * kHid, kActStride, kHid, nullptr, m.part, m.grad + kOffW4, m.grad + kOffB4, s);
