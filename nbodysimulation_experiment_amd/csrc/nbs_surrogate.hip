@@ -250,6 +250,21 @@ __device__ __forceinline__ PairSrc stage_pairs(const Buffers &b, const DevScene 
   return src;
 }
 
+// The source with its pointers re-derived from the LDS pool (tiled) or the
+// slot arrays, for code that handles one kind only.
+__device__ __forceinline__ PairSrc tile_src(PairSrc s, unsigned char *pool) {
+  s.x = reinterpret_cast<const float2 *>(pool);
+  s.pv = s.x + kTileCap;
+  s.d = s.pv + kTileCap;
+  return s;
+}
+__device__ __forceinline__ PairSrc slot_src(PairSrc s, const Buffers &b) {
+  s.x = b.x1;
+  s.pv = b.prev;
+  s.d = b.dens;
+  return s;
+}
+
 // Advances to the lane's next pair (j != i, r^2 < h^2); false when exhausted.
 // t is the pair's index in src.  Candidates are tested kWalkChunk at a time
 // into a mask, whose bits are then taken lowest first (candidate order).
@@ -394,32 +409,39 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
   }
   floatx16 h00 = zero16(), h01 = zero16(), h10 = zero16(), h11 = zero16();  // tile, hidden half
   uint32_t np = 0;
-  for (;;) {
-    uint32_t t = 0;
-    float2 xj = make_float2(0.0f, 0.0f);
-    float Rx = 0.0f, Ry = 0.0f, r2 = 0.0f;
-    const bool have = next_pair(cu, b, p, src, q, me, t, xj, Rx, Ry, r2);
-    if (__ballot(have) == 0ull) break;  // wave-uniform
-    uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-    if (have) {
-      pair_words(w, src, p, norm, q, t, xj, Rx, Ry, r2);
-      ++np;
+  auto pair_pass = [&](const PairSrc &sp) {
+    for (;;) {
+      uint32_t t = 0;
+      float2 xj = make_float2(0.0f, 0.0f);
+      float Rx = 0.0f, Ry = 0.0f, r2 = 0.0f;
+      const bool have = next_pair(cu, b, p, sp, q, me, t, xj, Rx, Ry, r2);
+      if (__ballot(have) == 0ull) break;  // wave-uniform
+      uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+      if (have) {
+        pair_words(w, sp, p, norm, q, t, xj, Rx, Ry, r2);
+        ++np;
+      }
+      const PairA a = pair_operands(w);
+      floatx16 zlo, zhi;
+      // K steps 0 and 1 of each tile and hidden half
+      pair_layer(a.t0, w1, zlo, zhi);
+      h00 = mfma16(sel[0], relu_rows(zlo, 0), h00);
+      h01 = mfma16(sel[0], relu_rows(zhi, 0), h01);
+      h00 = mfma16(sel[1], relu_rows(zlo, 1), h00);
+      h01 = mfma16(sel[1], relu_rows(zhi, 1), h01);
+      pair_layer(a.t1, w1, zlo, zhi);
+      h10 = mfma16(sel[0], relu_rows(zlo, 0), h10);
+      h11 = mfma16(sel[0], relu_rows(zhi, 0), h11);
+      h10 = mfma16(sel[1], relu_rows(zlo, 1), h10);
+      h11 = mfma16(sel[1], relu_rows(zhi, 1), h11);
     }
-    const PairA a = pair_operands(w);
-    floatx16 zlo, zhi;
-    pair_layer(a.t0, w1, zlo, zhi);
-#pragma unroll
-    for (int st = 0; st < 2; ++st) {
-      h00 = mfma16(sel[st], relu_rows(zlo, st), h00);
-      h01 = mfma16(sel[st], relu_rows(zhi, st), h01);
-    }
-    pair_layer(a.t1, w1, zlo, zhi);
-#pragma unroll
-    for (int st = 0; st < 2; ++st) {
-      h10 = mfma16(sel[st], relu_rows(zlo, st), h10);
-      h11 = mfma16(sel[st], relu_rows(zhi, st), h11);
-    }
-  }
+  };
+  // one copy of the pass per source, so that the tile's loads are LDS loads
+  // (a pointer that may be either is a flat address)
+  if (src.tiled)
+    pair_pass(tile_src(src, pool));
+  else
+    pair_pass(slot_src(src, b));
   shCnt[lane] = (float)np;
   if (m.pairPart) {
     uint32_t tot = np;
