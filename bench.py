@@ -148,7 +148,7 @@ def train_surrogate(args):
     from nbodysimulation_experiment_amd import scenes
     from nbodysimulation_experiment_amd.scenes import SPHParams
     from nbodysimulation_experiment_amd.sim import ParticleSimulation
-    specs = scenes.training_scenes(count=args.train_scenes)
+    specs = scenes.training_scenes(count=args.train_scenes, tall=args.train_tall)
     sim = ParticleSimulation.from_scenes(specs, SPHParams())
     sim.Update(DT, args.train_warm)
     sim.surrogate_init(0)
@@ -160,7 +160,8 @@ def train_surrogate(args):
     n = sim.GetParticleCount()
     sim.close()
     k = max(1, len(losses) // 10)
-    return params, {"scenes": len(specs), "particles": n, "warm_steps": args.train_warm, "steps": args.train_steps,
+    return params, {"scenes": len(specs), "tall_columns": args.train_tall, "particles": n,
+                    "warm_steps": args.train_warm, "steps": args.train_steps,
                     "learning_rate": args.lr, "seconds": round(t, 3),
                     "particle_steps_per_s": round(n * args.train_steps / t, 1),
                     "relative_error_first": float(rel[:k].mean()), "relative_error_last": float(rel[-k:].mean()),
@@ -299,7 +300,9 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="skip the learned-relaxation leg")
     ap.add_argument("--no-trajectory", action="store_true", help="skip the surrogate trajectory comparison")
     ap.add_argument("--no-graph-leg", action="store_true", help="skip the HIP-graph replay leg")
-    ap.add_argument("--train-scenes", type=int, default=200)
+    ap.add_argument("--train-scenes", type=int, default=168, help="16,384-particle training dam breaks")
+    ap.add_argument("--train-tall", type=int, default=8,
+                    help="64 x 1,024-particle training columns, as tall as the 1M dam break's")
     ap.add_argument("--train-warm", type=int, default=100)
     ap.add_argument("--train-steps", type=int, default=500)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -439,7 +442,8 @@ def main():
     surrogate = None
     online = args.workload == "batched-surrogate"  # in-run training on each check step's plain frame
     if args.workload in ("dam-break-1m", "batched-surrogate") and not args.no_surrogate and not args.graph:
-        log(f"training the learned relaxation: {args.train_scenes} scenes, {args.train_steps} steps")
+        log(f"training the learned relaxation: {args.train_scenes} scenes + {args.train_tall} tall columns, "
+            f"{args.train_steps} steps")
         params, train = train_surrogate(args)
         log(f"training: {train}")
 

@@ -223,11 +223,16 @@ def batched_scenes(seed: int = 2, count: int = 256, n_min: int = 2048, n_max: in
 
 
 def training_scenes(seed: int = 3, count: int = 200, n_side: int = 128, box_lo: float = 1.5,
-                    box_hi: float = 3.0) -> list[SceneSpec]:
+                    box_hi: float = 3.0, tall: int = 0, tall_shape: tuple[int, int] = (64, 1024),
+                    tall_box: tuple[float, float] = (160.0, 560.0)) -> list[SceneSpec]:
     """Frames for the learned relaxation: ``count`` dam breaks of n_side^2
     particles (16,384 by default, 0.5h spacing) resting in the lower-left
     corner of a box of width and height U[box_lo, box_hi] x the block side
-    each (rounded up to an even number of cells), 4 walls."""
+    each (rounded up to an even number of cells), 4 walls; then ``tall``
+    narrow columns of tall_shape particles (64 x 1,024 by default: 32h wide,
+    512h tall, as tall as the 1M dam break's column, whose bottom rows reach
+    densities that shallow columns never do) in the lower-left corner of a
+    tall_box box, 4 walls."""
     u = splitmix64_uniform(seed, 2 * count)
     block = n_side * 0.5
     scenes = []
@@ -241,5 +246,11 @@ def training_scenes(seed: int = 3, count: int = 200, n_side: int = 128, box_lo: 
         w, h = dims
         center = (-w * 0.5 + block * 0.5, -h * 0.5 + block * 0.5)
         pos = volume_positions(center, n_side, n_side, 0.5, 1.0, seed * 1_000_003 + s)
+        scenes.append(SceneSpec(w, h, 1.0, pos, planes=wall_planes(w, h)))
+    cx, cy = tall_shape
+    w, h = tall_box
+    for s in range(tall):
+        center = (-w * 0.5 + cx * 0.25, -h * 0.5 + cy * 0.25)
+        pos = volume_positions(center, cx, cy, 0.5, 1.0, seed * 1_000_003 + count + s)
         scenes.append(SceneSpec(w, h, 1.0, pos, planes=wall_planes(w, h)))
     return scenes

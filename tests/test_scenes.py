@@ -62,3 +62,22 @@ def test_batched_scenes_counts_and_boxes():
     assert 3.5e6 < n.sum() < 6.5e6
     for s in sp:
         assert pos_inside(s) and s.width == s.height and s.width % 2 == 0
+
+
+def test_training_scenes_with_tall_columns():
+    """168 shallow dam breaks plus 8 columns as tall as the 1M scene's: the
+    bench's training set of 3,276,800 particles, every block inside its box."""
+    specs = scenes.training_scenes(count=168, tall=8)
+    assert len(specs) == 176
+    assert sum(s.n for s in specs) == 3_276_800
+    for s in specs[:168]:
+        assert s.n == 128 * 128
+    for s in specs[168:]:
+        assert s.n == 64 * 1024
+        lo, hi = s.positions.min(0), s.positions.max(0)
+        assert hi[1] - lo[1] > 511.0 and hi[0] - lo[0] < 32.0  # 512h tall, 32h wide
+        assert lo[0] > -s.width / 2 and lo[1] > -s.height / 2 and hi[0] < s.width / 2 and hi[1] < s.height / 2
+    # the shallow scenes are those of the plain set, and the columns differ by seed
+    plain = scenes.training_scenes(count=168)
+    assert all(np.array_equal(a.positions, b.positions) for a, b in zip(specs[:168], plain))
+    assert not np.array_equal(specs[168].positions, specs[169].positions)
