@@ -70,13 +70,21 @@ def kernel_stats(path: str) -> list[dict]:
 
 
 def counters(path: str) -> dict:
-    """{kernel: {counter: mean per dispatch}} from a counter_collection CSV."""
+    """{kernel: {counter: mean per dispatch}} from a counter_collection CSV.
+    A kernel launched with several grid sizes (the learned relaxation's
+    forward at the timed scene and at the training set) keeps its name for
+    the largest grid; the others are listed as "name@grid"."""
     acc = collections.defaultdict(lambda: collections.defaultdict(list))
     for r in csv.DictReader(open(path)):
         name = short_name(r.get("Kernel_Name", r.get("Kernel-Name", "")))
-        acc[name][r["Counter_Name"]].append(float(r["Counter_Value"]))
-    return {k: {c: sum(v) / len(v) for c, v in d.items()} | {"dispatches": max(len(v) for v in d.values())}
-            for k, d in acc.items()}
+        grid = int(float(r.get("Grid_Size", 0) or 0))
+        acc[(name, grid)][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    largest = {}
+    for name, grid in acc:
+        largest[name] = max(largest.get(name, grid), grid)
+    return {(name if grid == largest[name] else f"{name}@{grid}"):
+            {c: sum(v) / len(v) for c, v in d.items()} | {"dispatches": max(len(v) for v in d.values())}
+            for (name, grid), d in acc.items()}
 
 
 def _per_dispatch(path: str, counter: str) -> dict:
