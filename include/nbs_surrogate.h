@@ -29,6 +29,15 @@
  *       max_density_error.  If any scene's error is above it, the plain
  *       result replaces the surrogate's for that step (in every scene) and
  *       the next check_interval steps run the plain method (fallback).
+ *   NBS_RELAX_SURROGATE_CARRIED_DENSITY   as NBS_RELAX_SURROGATE, but the
+ *       network steps between checks run no density pass: their features take
+ *       each particle's density in the result the last check accepted,
+ *       carried with the particle through the binning.  Between checks the
+ *       state's density is that carried density, its pressure is computed
+ *       from it (sph.h:478-481), and the candidate-count statistics are those
+ *       of the last density pass.  Steps that change the particle set or the
+ *       state, and plain steps, drop the carried density; the next check
+ *       sets it again.
  */
 #ifndef NBS_SURROGATE_H
 #define NBS_SURROGATE_H
@@ -42,7 +51,7 @@ extern "C" {
 #define NBS_SURROGATE_WEIGHT_COUNT 9474
 #define NBS_SURROGATE_PARAM_COUNT 9482
 
-enum { NBS_RELAX_PLAIN = 0, NBS_RELAX_SURROGATE = 1 };
+enum { NBS_RELAX_PLAIN = 0, NBS_RELAX_SURROGATE = 1, NBS_RELAX_SURROGATE_CARRIED_DENSITY = 2 };
 
 typedef struct nbs_surrogate_report {
   uint64_t surrogate_steps;   /* steps relaxed by the network (check steps that passed included) */

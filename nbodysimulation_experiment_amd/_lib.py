@@ -19,6 +19,7 @@ SURROGATE_WEIGHT_COUNT = 9474
 SURROGATE_PARAM_COUNT = 9482
 RELAX_PLAIN = 0
 RELAX_SURROGATE = 1
+RELAX_SURROGATE_CARRIED_DENSITY = 2
 
 c_float, c_int32, c_uint32, c_uint64, c_double = (ctypes.c_float, ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint64,
                                                   ctypes.c_double)

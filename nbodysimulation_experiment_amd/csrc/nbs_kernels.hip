@@ -546,6 +546,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
   b.prev[d] = b.x0[s];  // demo4.cpp:334 prevPosition = curPosition
   b.id2[d] = b.id[s];
   b.keys[d] = key;
+  if (b.densTo) b.densTo[d] = b.densFrom[s];
 }
 
 // ---------------------------------------------------------------------------
@@ -617,6 +618,7 @@ __device__ __forceinline__ void place_move(const Buffers &b, uint32_t s, uint32_
   b.prev[d] = b.x0[s];  // demo4.cpp:334 prevPosition = curPosition
   b.id2[d] = b.id[s];
   b.keys[d] = k;
+  if (b.densTo) b.densTo[d] = b.densFrom[s];
 }
 
 template <bool MULTI>

@@ -95,6 +95,10 @@ struct Buffers {
   uint32_t *farCount;       // entries of farList this step (zero when the step starts)
   uint32_t *farCountClear;  // the other step parity's counter: cleared by the cell scan
   uint32_t *chunkCells;     // per kPlaceChunk new slots: the cells holding its first and last slot
+  // If set, the binning also moves densFrom (previous slot order) into densTo
+  // (new slot order): a per-particle value carried along with its particle.
+  const float2 *densFrom;
+  float2 *densTo;
 };
 
 constexpr uint32_t kPlaceChunk = 1024;  // new slots per placement workgroup

@@ -204,6 +204,7 @@ class Simulation {
     NBS_HIP(hipStreamSynchronize(stream_));
     const uint64_t first = n_;
     n_ += k;
+    dropCarriedDensity();
     destroyGraph();
     return first;
   }
@@ -211,6 +212,7 @@ class Simulation {
     n_ = 0;
     nBinned_ = 0;
     lists_ = false;
+    dropCarriedDensity();
     destroyGraph();
   }
   // demo4.cpp:169-181 AddVolume with seeded per-axis jitter.
@@ -299,6 +301,7 @@ class Simulation {
     NBS_HIP(hipStreamSynchronize(stream_));
     nBinned_ = 0;
     lists_ = false;
+    dropCarriedDensity();
     destroyGraph();
   }
   void getGrid(uint32_t *sortedIds, uint32_t *cellStart) {
@@ -408,7 +411,11 @@ class Simulation {
         ensureEvents(slots + 1);
         ev = (int)slots++;
       }
-      const int relax = surrogateMode() ? nextRelax() : RELAX_PLAIN;
+      int relax = surrogateMode() ? nextRelax() : RELAX_PLAIN;
+      if (relax == RELAX_SURROGATE && carryReady()) relax = RELAX_SURROGATE_CARRIED;
+      // a step that does not move the carried density leaves it in an old slot
+      // order (a check step sets it afresh)
+      if (relax != RELAX_SURROGATE_CARRIED) dropCarriedDensity();
       if (relax != RELAX_PLAIN) {
         ensureSurInference();
         ensureSceneErr();
@@ -501,8 +508,9 @@ class Simulation {
     NBS_HIP(hipStreamSynchronize(stream_));
   }
   void surSetMode(int mode, uint32_t interval, float maxErr) {
-    if (mode != NBS_RELAX_PLAIN && mode != NBS_RELAX_SURROGATE) throw Error(NBS_ERR_INVALID_ARGUMENT, "unknown mode");
-    if (mode == NBS_RELAX_SURROGATE) {
+    if (mode != NBS_RELAX_PLAIN && mode != NBS_RELAX_SURROGATE && mode != NBS_RELAX_SURROGATE_CARRIED_DENSITY)
+      throw Error(NBS_ERR_INVALID_ARGUMENT, "unknown mode");
+    if (mode != NBS_RELAX_PLAIN) {
       requireSur();
       if (interval == 0) throw Error(NBS_ERR_INVALID_ARGUMENT, "check_interval must be at least 1");
       if (!(maxErr >= 0.0f) || !std::isfinite(maxErr))
@@ -520,6 +528,7 @@ class Simulation {
     S.rep.last_max_scene_error = 0.0;
     S.rep.last_scenes_over_limit = 0;
     S.errSum = 0.0;
+    S.carrySrc = nullptr;
     destroyGraph();
   }
   void surSetOnlineTraining(float lr) {
@@ -748,6 +757,7 @@ class Simulation {
     }
     nBinned_ = 0;
     lists_ = false;
+    dropCarriedDensity();
     destroyGraph();
   }
   void ensureStaging() {
@@ -872,13 +882,20 @@ class Simulation {
     bg.chunkCells = place ? d_.chunkCells.p : nullptr;
     bg.farCount = d_.farCounts.p + parity_;
     bg.farCountClear = d_.farCounts.p + (parity_ ^ 1u);
+    const bool carried = relax == RELAX_SURROGATE_CARRIED;
+    if (carried) {
+      bg.densFrom = sur_->carrySrc;
+      bg.densTo = sur_->carry.p;
+    }
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 0], stream_));
     launch_visc_predict(bg, p, s0, n, (int)nBinned_, lists_ ? 1 : 0, cap[0], totalCells_, stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 1], stream_));
     cellSlots_ = launch_grid(bg, s0, totalCells_, n, cap[0], place, stream_);
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 2], stream_));
-    launch_density(buf_, p, s0, n, cap[1], stream_);
-    statSlots_ = density_stat_slots(n, cap[1]);
+    if (!carried) {
+      launch_density(buf_, p, s0, n, cap[1], stream_);
+      statSlots_ = density_stat_slots(n, cap[1]);
+    }
     if (ev >= 0) NBS_HIP(hipEventRecord(events_[5 * ev + 3], stream_));
     if (relax == RELAX_PLAIN) {
       launch_delta(buf_, p, s0, n, cap[2], stream_);
@@ -1081,7 +1098,7 @@ class Simulation {
 
   // ---- learned relaxation (internals) ----------------------------------------
   enum Relax { RELAX_PLAIN = 0, RELAX_SURROGATE = 1, RELAX_CHECK = 2, RELAX_TRAIN = 3, RELAX_FRAME = 4,
-               RELAX_CALIBRATE = 5 };
+               RELAX_CALIBRATE = 5, RELAX_SURROGATE_CARRIED = 6 };
 
   struct SurrogateState {
     DevBuf<float> params, adamM, adamV, grad, out, act, cnt, dH, dG, dM, dO, part, lossPart, loss;
@@ -1102,9 +1119,23 @@ class Simulation {
     uint32_t plainLeft = 0;
     nbs_surrogate_report rep{};
     double errSum = 0.0;
+    // Carried density (NBS_RELAX_SURROGATE_CARRIED_DENSITY): carrySrc is the
+    // density the last check accepted (then the state's density), in the
+    // previous step's slot order; the binning of each network step moves it
+    // into `carry`, which then becomes the state's density array.
+    DevBuf<float2> carry;
+    const float2 *carrySrc = nullptr;
+    uint64_t carryN = 0;
   };
 
-  bool surrogateMode() const { return sur_ && sur_->mode == NBS_RELAX_SURROGATE; }
+  bool surrogateMode() const { return sur_ && sur_->mode != NBS_RELAX_PLAIN; }
+  bool carryReady() const {
+    return sur_->mode == NBS_RELAX_SURROGATE_CARRIED_DENSITY && sur_->carrySrc && sur_->carryN == n_ && lists_ &&
+           nBinned_ == n_ && emitters_.empty();
+  }
+  void dropCarriedDensity() {
+    if (sur_) sur_->carrySrc = nullptr;
+  }
   void requireSur() const {
     if (!sur_) throw Error(NBS_ERR_INVALID_ARGUMENT, "the surrogate is not initialised (nbs_surrogate_init)");
   }
@@ -1129,6 +1160,8 @@ class Simulation {
     S.errPartials.alloc(2 * std::max<size_t>((c + 2047) / 2048, 1));
     S.errOut.alloc(2);
     S.pairPart.alloc(surrogate_forward_waves((int)c));
+    S.carry.alloc(c);
+    S.carrySrc = nullptr;
     S.inferCap = c;
   }
   void ensureSceneErr() {
@@ -1190,6 +1223,18 @@ class Simulation {
       case RELAX_SURROGATE:
         launch_surrogate_forward(buf_, p, s0, n, surBufs(false, 0), kFwdState, stream_);
         break;
+      case RELAX_SURROGATE_CARRIED: {
+        // the binning moved the carried density into S.carry; the network
+        // reads it there and writes the matching pressures, and it becomes
+        // the state's density array (the old one is the next step's target)
+        Buffers bc = buf_;
+        bc.dens = S.carry.p;
+        launch_surrogate_forward(bc, p, s0, n, surBufs(false, 0), kFwdState | kFwdPressure, stream_);
+        d_.dens.swap(S.carry);
+        buf_.dens = d_.dens.p;
+        S.carrySrc = buf_.dens;
+        break;
+      }
       case RELAX_CHECK: {
         // plain result into the shadow state, the network's into the state;
         // with online training the plain displacement is also the target
@@ -1285,7 +1330,8 @@ class Simulation {
     S.rep.last_max_scene_error = worst;
     S.rep.last_scenes_over_limit = over;
     S.errSum += err;
-    if (!(err <= (double)S.maxErr) || over > 0) {
+    const bool fallback = !(err <= (double)S.maxErr) || over > 0;
+    if (fallback) {
       NBS_HIP(hipMemcpyAsync(buf_.x0, S.shadowX.p, n_ * sizeof(float2), hipMemcpyDeviceToDevice, stream_));
       NBS_HIP(hipMemcpyAsync(buf_.v, S.shadowV.p, n_ * sizeof(float2), hipMemcpyDeviceToDevice, stream_));
       ++S.rep.fallbacks;
@@ -1294,6 +1340,9 @@ class Simulation {
     } else {
       ++S.rep.surrogate_steps;
     }
+    // the density of the accepted result, on this step's slot order
+    S.carrySrc = fallback ? S.densB.p : S.densA.p;
+    S.carryN = n_;
   }
   // One plain step that also captures the frame (relax: RELAX_CALIBRATE or
   // RELAX_FRAME); copies the requested slot-order arrays to the host.

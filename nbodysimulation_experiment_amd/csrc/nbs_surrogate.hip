@@ -531,6 +531,10 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
       }
       collide_wave<MULTI>(b, p, sc, x, y, head);                               // demo4.cpp:411-444
       if (head) {
+        if (mode & kFwdPressure) {
+          const float2 d = b.dens[i];
+          b.pres[i] = make_float2(p.k * (d.x - p.rho0), p.kn * d.y);           // sph.h:478-481
+        }
         const float2 pv = b.prev[i];
         b.x0[i] = make_float2(x, y);
         b.v[i] = make_float2((x - pv.x) * p.invDt, (y - pv.y) * p.invDt);      // demo4.cpp:446-450

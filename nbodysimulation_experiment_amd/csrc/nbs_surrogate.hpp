@@ -75,6 +75,7 @@ struct SurrogateBufs {
 // writes the training activations (out, act, cnt); out is written when set.
 constexpr int kFwdState = 1;
 constexpr int kFwdTrain = 2;
+constexpr int kFwdPressure = 4;  // with kFwdState: also pres = (k (rho - rho0), k_near rho_near) from b.dens
 void launch_surrogate_forward(const Buffers &b, const DevParams &p, const DevScene &s0, int n,
                               const SurrogateBufs &m, int mode, hipStream_t stream);
 // Gradient of L for the frame of the last training forward into m.grad

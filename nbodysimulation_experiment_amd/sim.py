@@ -324,8 +324,14 @@ class ParticleSimulation:
         call("nbs_surrogate_train", self._h, float(dt), int(steps), float(learning_rate), _ptr(losses), _ptr(rel))
         return (losses[:int(steps)], rel[:int(steps)]) if relative else losses[:int(steps)]
 
-    def surrogate_set_mode(self, surrogate: bool, check_interval: int = 8, max_density_error: float = 0.01):
-        mode = _lib.RELAX_SURROGATE if surrogate else _lib.RELAX_PLAIN
+    def surrogate_set_mode(self, surrogate: bool, check_interval: int = 8, max_density_error: float = 0.01,
+                           carried_density: bool = False):
+        """The network relaxes every step (surrogate=True), checked every
+        check_interval steps; with carried_density the steps between checks
+        skip the density pass and take the last check's density."""
+        mode = _lib.RELAX_PLAIN
+        if surrogate:
+            mode = _lib.RELAX_SURROGATE_CARRIED_DENSITY if carried_density else _lib.RELAX_SURROGATE
         call("nbs_surrogate_set_mode", self._h, mode, int(check_interval), float(max_density_error))
 
     def surrogate_set_online_training(self, learning_rate: float):
