@@ -412,15 +412,12 @@ class Simulation {
         ev = (int)slots++;
       }
       int relax = surrogateMode() ? nextRelax() : RELAX_PLAIN;
-      if (relax == RELAX_SURROGATE && carryReady()) relax = RELAX_SURROGATE_CARRIED;
-      // a step that does not move the carried density leaves it in an old slot
-      // order (a check step sets it afresh)
-      if (relax != RELAX_SURROGATE_CARRIED) dropCarriedDensity();
       if (relax != RELAX_PLAIN) {
         ensureSurInference();
         ensureSceneErr();
         if (relax == RELAX_CHECK && sur_->onlineLr > 0.0f) ensureSurTraining();
       }
+      if (relax == RELAX_SURROGATE && carryReady()) relax = RELAX_SURROGATE_CARRIED;
       enqueueStep(dt, ev, relax);
       swapAfterStep();
       if (relax == RELAX_CHECK) finishCheck();
@@ -868,6 +865,9 @@ class Simulation {
   // One step on stream_ (ev >= 0: record timing events for step slot ev);
   // relax selects how the relaxation phase runs (Relax).
   void enqueueStep(float dt, int ev, int relax = RELAX_PLAIN, int lossIndex = 0) {
+    // every step that does not move the carried density leaves it in an old
+    // slot order (a check step sets it afresh when it finishes)
+    if (relax != RELAX_SURROGATE_CARRIED) dropCarriedDensity();
     const DevParams p = devParams(dt);
     const DevScene s0 = hostScenes_[0];
     const int n = (int)n_;
