@@ -319,7 +319,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build computes on one GPU; runs on more than one GPU are not supported", file=sys.stderr)
+        print("bench.py: the library computes on one GPU; runs on more than one GPU are not supported", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "scenarios":

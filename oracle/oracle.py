@@ -1,8 +1,8 @@
 """ctypes wrapper of the CPU oracle.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker or the CPU baseline, never
-as the measured GPU path.
+cpu_baseline leg of bench.py, always as the reference results the GPU path is
+compared with or as the CPU baseline, never as the measured GPU path.
 """
 from __future__ import annotations
 

@@ -1,8 +1,8 @@
 """CPU oracle of the learned relaxation (test infrastructure only).
 
 The product path is the HIP library (nbodysimulation_experiment_amd/csrc/
-nbs_surrogate.hip); this module is the checker the tests compare it with and
-is never called by the product.  The network has no counterpart in the
+nbs_surrogate.hip); this module computes the reference results the tests
+compare it with and is never called by the product.  The network has no counterpart in the
 reference: it stands in for the double-density relaxation of
 NBodySimulation/demo4.cpp:239-255 (sph.h:483-495), whose displacement is its
 training target.

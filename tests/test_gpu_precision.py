@@ -8,6 +8,7 @@ through step 100 for the 4,096-particle dam break.
 """
 import json
 import os
+import tempfile
 
 import numpy as np
 import pytest
@@ -16,7 +17,8 @@ from nbodysimulation_experiment_amd.scenes import dam_break_1m, dam_break_4k
 from tests.helpers import DT, gpu_for, max_pos_err, oracle_for
 
 pytestmark = pytest.mark.gpu
-OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+# error curves are written to NBS_TEST_OUT, by default a temporary folder
+OUT = os.environ.get("NBS_TEST_OUT", os.path.join(tempfile.gettempdir(), "nbs_test_out"))
 
 
 def _curve(spec, marks):

@@ -13,6 +13,7 @@ from __future__ import annotations
 import json
 import os
 import subprocess
+import tempfile
 
 import numpy as np
 import pytest
@@ -26,7 +27,8 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "nbodysimulation_experiment_amd", "nbs_benchmark")
-OUT = os.path.join(ROOT, "gpurun_out")
+# result files (per-scenario tables) go to NBS_TEST_OUT, by default a temporary folder
+OUT = os.environ.get("NBS_TEST_OUT", os.path.join(tempfile.gettempdir(), "nbs_test_out"))
 FRAMES = 64
 
 
