@@ -265,12 +265,13 @@ __device__ __forceinline__ void viscosity_sum(const uint32_t *start, const DevSc
         const float qq = r * p.invH;                      // sph.h:504
         const float nX = Rx * inv, nY = Ry * inv;
         const float u = (vx - vjx) * nX + (vy - vjy) * nY;  // sph.h:506
-        const float f = (1.0f - qq) * (p.sigma * u + p.beta * (u * u));  // sph.h:508
+        // no impulse unless u > 0 (sph.h:507): a zero force times the finite
+        // normal, 0.5 and dt gives +0 or -0, and subtracting either leaves the
+        // sums unchanged (they start at +0 and never hold -0)
+        const float f = (use && u > 0.0f) ? (1.0f - qq) * (p.sigma * u + p.beta * (u * u)) : 0.0f;  // sph.h:508
         const float ix = ((f * nX) * 0.5f) * p.dt;       // demo4.cpp:233 force * 0.5f * deltaTime
         const float iy = ((f * nY) * 0.5f) * p.dt;
-        // no impulse unless u > 0 (sph.h:507): subtracting +0 leaves the sums unchanged
-        const bool on = use && u > 0.0f;
-        return make_float2(on ? ix : 0.0f, on ? iy : 0.0f);
+        return make_float2(ix, iy);
       },
       [&](float2 im) {
         accx = accx - im.x;                               // own visit
@@ -774,10 +775,11 @@ __device__ __forceinline__ uint32_t density_sum(const uint32_t *start, const Dev
       start, sc, cx, cy, X, idx, xi, p.h2,
       [&](uint32_t, float, float, float r2, bool use) {
         const float r = pair_sqrt(r2);                    // sph.h:471
-        const float term = 1.0f - r * p.invH;             // sph.h:472
+        // term +0 gives terms +0: adding them leaves the sums unchanged (they
+        // start at +0 and never hold -0)
+        const float term = use ? 1.0f - r * p.invH : 0.0f;  // sph.h:472
         const float t2 = term * term, t3 = (term * term) * term;
-        // adding -0 leaves the sums unchanged
-        return make_float2(use ? t2 : -0.0f, use ? t3 : -0.0f);
+        return make_float2(t2, t3);
       },
       [&](float2 w) {
         d0 = d0 + w.x;                                    // sph.h:473
@@ -850,11 +852,12 @@ __device__ __forceinline__ void delta_sum(const uint32_t *start, const DevScene 
         const float nX = Rx * inv, nY = Ry * inv;
         const float term = 1.0f - r * p.invH;             // sph.h:491
         const float tt = term * term;
-        const float dij = p.dt2 * (pi.x * term + pi.y * tt);  // sph.h:492 with P_i
-        const float dji = p.dt2 * (pj.x * term + pj.y * tt);  // sph.h:492 with P_j (j's visit)
-        // subtracting +0 leaves the sums unchanged
-        return make_float4(use ? (dij * nX) * 0.5f : 0.0f, use ? (dij * nY) * 0.5f : 0.0f,
-                           use ? (dji * nX) * 0.5f : 0.0f, use ? (dji * nY) * 0.5f : 0.0f);
+        // a zero displacement times the finite normal and 0.5 gives +0 or -0,
+        // and subtracting either leaves the sums unchanged (they start at +0
+        // and never hold -0)
+        const float dij = use ? p.dt2 * (pi.x * term + pi.y * tt) : 0.0f;  // sph.h:492 with P_i
+        const float dji = use ? p.dt2 * (pj.x * term + pj.y * tt) : 0.0f;  // sph.h:492 with P_j (j's visit)
+        return make_float4((dij * nX) * 0.5f, (dij * nY) * 0.5f, (dji * nX) * 0.5f, (dji * nY) * 0.5f);
       },
       [&](float4 d) {
         accx = accx - d.x;                                // demo4.cpp:251 dx -= delta * 0.5f
