@@ -185,6 +185,9 @@ struct Cursor {
   int d;
   bool done;
   uint32_t mask, mbase;  // in-range candidates of the current chunk: bit b is slot mbase + b
+  // rows tested up front (cursor_rows): the masks and source bases of the
+  // rows after the current one
+  uint32_t m1, b1, m2, b2;
 };
 
 // Neighbour data of a wave's candidates: an LDS copy of the three candidate
@@ -265,13 +268,86 @@ __device__ __forceinline__ PairSrc slot_src(PairSrc s, const Buffers &b) {
   return s;
 }
 
+// Cut-off mask of the cnt <= 32 candidates at xs[0 .. cnt): bit k is set when
+// candidate k lies within h of xi.
+__device__ __forceinline__ uint32_t chunk_bits(const float2 *xs, uint32_t cnt, float2 xi, float h2) {
+  // m = 2 * m + (d2 < h2), last candidate first, so candidate k ends at bit
+  // k: the compare's lane mask is the add's carry-in
+  uint32_t m = 0;
+  auto push = [&](float d2) {
+    asm("v_cmp_gt_f32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc" : "+v"(m) : "s"(h2), "v"(d2) : "vcc");
+  };
+  auto dist2 = [&](uint32_t k) {
+    const float2 x = xs[k];
+    const float dx = x.x - xi.x, dy = x.y - xi.y;
+    return dx * dx + dy * dy;
+  };
+  uint32_t u = cnt;
+  for (; u >= 8u; u -= 8u) {  // eight position loads in flight
+    float d2[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) d2[k] = dist2(u - 1 - k);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) push(d2[k]);
+  }
+  for (; u >= 4u; u -= 4u) {
+    const float a0 = dist2(u - 1), a1 = dist2(u - 2), a2 = dist2(u - 3), a3 = dist2(u - 4);
+    push(a0);
+    push(a1);
+    push(a2);
+    push(a3);
+  }
+  for (; u > 0u; --u) push(dist2(u - 1));
+  return m;
+}
+
+// Rows of at most kWalkRowMax candidates can be tested up front: when every
+// lane's three rows fit (rows_fit, wave-uniform), cursor_rows tests all of
+// them with the wave's lanes together and next_pair then only moves from one
+// row's mask to the next.  Otherwise a lane that runs out of marked
+// candidates tests its next chunk of kWalkChunk while the rest of the wave
+// waits, once per chunk of each lane.
+constexpr uint32_t kWalkChunk = 16;
+constexpr uint32_t kWalkRowMax = 32;
+
+__device__ __forceinline__ bool rows_fit(const Buffers &b, const PartInfo &q, bool live) {
+  bool fit = true;
+  if (live)
+    for (int dy = -1; dy <= 1; ++dy) {
+      const RowRange rr = row_range(b.startNext, q.sc, q.cx, q.cy, dy);
+      fit = fit && rr.j1 - rr.j0 <= kWalkRowMax;
+    }
+  return __ballot(!fit) == 0ull;
+}
+
+// Tests the lane's three rows (every row at most kWalkRowMax candidates) and
+// loads their masks into the cursor; the particle itself is no pair.
+__device__ __forceinline__ void cursor_rows(Cursor &cu, const Buffers &b, const DevParams &p, const PairSrc &src,
+                                            const PartInfo &q, int i) {
+  uint32_t m[3], base[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const RowRange rr = row_range(b.startNext, q.sc, q.cx, q.cy, r - 1);
+    const uint32_t cnt = rr.j1 - rr.j0;
+    // the row's first index in the source: j0 + base(r) wraps modulo 2^32
+    // (off - rowStart on the LDS tile), so it is formed in 32 bits first
+    base[r] = (uint32_t)(rr.j0 + src.base(r));
+    m[r] = chunk_bits(src.x + base[r], cnt, q.xi, p.h2);
+    const uint32_t self = (uint32_t)i - rr.j0;
+    if (self < cnt) m[r] &= ~(1u << self);
+  }
+  cu.mask = m[0], cu.mbase = base[0], cu.tb = 0u;
+  cu.m1 = m[1], cu.b1 = base[1], cu.m2 = m[2], cu.b2 = base[2];
+}
+
 // Advances to the lane's next pair (j != i, r^2 < h^2); false when exhausted.
 // t is the pair's index in src.  Candidates are tested kWalkChunk at a time
-// into a mask, whose bits are then taken lowest first (candidate order).
-constexpr uint32_t kWalkChunk = 16;
+// into a mask, whose bits are then taken lowest first (candidate order); with
+// rows (the cursor loaded by cursor_rows) the masks of all three rows are
+// there already.
 __device__ __forceinline__ bool next_pair(Cursor &cu, const Buffers &b, const DevParams &p, const PairSrc &src,
-                                          const PartInfo &q, int i, uint32_t &t, float2 &xj, float &Rx, float &Ry,
-                                          float &r2) {
+                                          const PartInfo &q, int i, bool rows, uint32_t &t, float2 &xj, float &Rx,
+                                          float &Ry, float &r2) {
   for (;;) {
     if (cu.mask) {
       const uint32_t bit = (uint32_t)__builtin_ctz(cu.mask);
@@ -284,41 +360,24 @@ __device__ __forceinline__ bool next_pair(Cursor &cu, const Buffers &b, const De
       return true;
     }
     if (cu.done) return false;
+    if (rows) {
+      if ((cu.m1 | cu.m2) == 0u) {
+        cu.done = true;
+        return false;
+      }
+      cu.mask = cu.m1, cu.mbase = cu.b1;
+      cu.m1 = cu.m2, cu.b1 = cu.b2;
+      cu.m2 = 0u;
+      continue;
+    }
     if (cu.j >= cu.jend && !cursor_next_row(cu, b.startNext, q, src)) {
       cu.done = true;
       return false;
     }
     const uint32_t cnt = min(kWalkChunk, cu.jend - cu.j);
-    // m = 2 * m + (d2 < h2), last candidate first, so candidate k ends at bit
-    // k: the compare's lane mask is the add's carry-in
-    uint32_t m = 0;
     // the chunk's first index in the source: j + tb wraps modulo 2^32 (tb is
     // off - rowStart on the LDS tile), so it is formed in 32 bits first
-    const float2 *xs = src.x + (uint32_t)(cu.j + cu.tb);
-    auto push = [&](float d2) {
-      asm("v_cmp_gt_f32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc" : "+v"(m) : "s"(p.h2), "v"(d2) : "vcc");
-    };
-    auto dist2 = [&](uint32_t k) {
-      const float2 x = xs[k];
-      const float dx = x.x - q.xi.x, dy = x.y - q.xi.y;
-      return dx * dx + dy * dy;
-    };
-    uint32_t u = cnt;
-    for (; u >= 8u; u -= 8u) {  // eight position loads in flight
-      float d2[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) d2[k] = dist2(u - 1 - k);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) push(d2[k]);
-    }
-    for (; u >= 4u; u -= 4u) {
-      const float a0 = dist2(u - 1), a1 = dist2(u - 2), a2 = dist2(u - 3), a3 = dist2(u - 4);
-      push(a0);
-      push(a1);
-      push(a2);
-      push(a3);
-    }
-    for (; u > 0u; --u) push(dist2(u - 1));
+    uint32_t m = chunk_bits(src.x + (uint32_t)(cu.j + cu.tb), cnt, q.xi, p.h2);
     const uint32_t self = (uint32_t)i - cu.j;  // the particle itself is no pair
     if (self < cnt) m &= ~(1u << self);
     cu.mbase = cu.j;
@@ -410,11 +469,15 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
   floatx16 h00 = zero16(), h01 = zero16(), h10 = zero16(), h11 = zero16();  // tile, hidden half
   uint32_t np = 0;
   auto pair_pass = [&](const PairSrc &sp) {
+    // rows up front in the single-scene forward only: on the multi-scene
+    // training batches they measured slower (forward and pair backward)
+    const bool rows = !MULTI && rows_fit(b, q, me < n);
+    if (rows && me < n) cursor_rows(cu, b, p, sp, q, me);
     for (;;) {
       uint32_t t = 0;
       float2 xj = make_float2(0.0f, 0.0f);
       float Rx = 0.0f, Ry = 0.0f, r2 = 0.0f;
-      const bool have = next_pair(cu, b, p, sp, q, me, t, xj, Rx, Ry, r2);
+      const bool have = next_pair(cu, b, p, sp, q, me, rows, t, xj, Rx, Ry, r2);
       if (__ballot(have) == 0ull) break;  // wave-uniform
       uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
       if (have) {
@@ -790,7 +853,7 @@ __global__ __launch_bounds__(64) void k_sur_back_pairs(Buffers b, DevParams p, D
       uint32_t t = 0;
       float2 xj = make_float2(0.0f, 0.0f);
       float Rx = 0.0f, Ry = 0.0f, r2 = 0.0f;
-      const bool have = next_pair(cu, b, p, src, q, me, t, xj, Rx, Ry, r2);
+      const bool have = next_pair(cu, b, p, src, q, me, false, t, xj, Rx, Ry, r2);
       if (__ballot(have) == 0ull) break;  // wave-uniform
       uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
       if (have) pair_words(w, src, p, norm, q, t, xj, Rx, Ry, r2);
