@@ -469,9 +469,9 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
   floatx16 h00 = zero16(), h01 = zero16(), h10 = zero16(), h11 = zero16();  // tile, hidden half
   uint32_t np = 0;
   auto pair_pass = [&](const PairSrc &sp) {
-    // rows up front in the single-scene forward only: on the multi-scene
-    // training batches they measured slower (forward and pair backward)
-    const bool rows = !MULTI && rows_fit(b, q, me < n);
+    // rows up front except for the multi-scene training forward, where they
+    // measured slower
+    const bool rows = !(MULTI && (mode & kFwdTrain)) && rows_fit(b, q, me < n);
     if (rows && me < n) cursor_rows(cu, b, p, sp, q, me);
     for (;;) {
       uint32_t t = 0;
