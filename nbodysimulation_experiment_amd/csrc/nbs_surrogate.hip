@@ -440,6 +440,18 @@ __device__ __forceinline__ void pair_layer(bf16x8 a, const W1Operands &w, floatx
 // ---------------------------------------------------------------------------
 // Forward
 // ---------------------------------------------------------------------------
+// Coalesced copy of a 32-row head tile into one block of the training
+// activations (rows row0 .. row0 + 31, columns col .. col + 63 of the
+// kActStride-wide rows): 16 lanes store one row's 64 values as float4, 256
+// contiguous bytes, so one store instruction covers four rows.
+template <typename Load4>
+__device__ __forceinline__ void store_act_rows(float *act, int col, int row0, int n, Load4 &&load4) {
+  for (int q = (int)threadIdx.x; q < 32 * 16; q += 64) {
+    const int r = q >> 4, c4 = (q & 15) * 4;
+    if (row0 + r < n) *reinterpret_cast<float4 *>(act + (size_t)(row0 + r) * kActStride + col + c4) = load4(r, c4);
+  }
+}
+
 template <bool MULTI>
 __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, DevScene s0, int n, SurrogateBufs m,
                                                     int mode) {
@@ -540,10 +552,12 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
     }
     __syncthreads();
     const int i = base + 32 * t + c;  // A-operand row c of this tile
-    if (train && g == 0 && i < n) {
-      float *act = m.act + (size_t)i * kActStride;
-      for (int k = 0; k < kHid; ++k) act[k] = bf2f(hb[c][k]);
-      m.cnt[i] = shCnt[32 * t + c];
+    auto hb4 = [&](int r, int c4) {
+      return make_float4(bf2f(hb[r][c4]), bf2f(hb[r][c4 + 1]), bf2f(hb[r][c4 + 2]), bf2f(hb[r][c4 + 3]));
+    };
+    if (train) {
+      store_act_rows(m.act, 0, base + 32 * t, n, hb4);
+      if (g == 0 && i < n) m.cnt[i] = shCnt[32 * t + c];
     }
     // M = W2 H + n b2, kept as BF16 in hb
     floatx16 olo, ohi;
@@ -557,10 +571,7 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
       hb[row][32 + c] = f2bf(ohi[k] + nc * prm[kOffB2 + 32 + c]);
     }
     __syncthreads();
-    if (train && g == 0 && i < n) {
-      float *act = m.act + (size_t)i * kActStride + kHid;
-      for (int k = 0; k < kHid; ++k) act[k] = bf2f(hb[c][k]);
-    }
+    if (train) store_act_rows(m.act, kHid, base + 32 * t, n, hb4);
     // G = relu(W3 M + b3), FP32
     dense(m.w3b, olo, ohi);
 #pragma unroll
@@ -570,10 +581,10 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
       gbuf[row][32 + c] = relu(ohi[k] + prm[kOffB3 + 32 + c]);
     }
     __syncthreads();
-    if (train && g == 0 && i < n) {
-      float *act = m.act + (size_t)i * kActStride + 2 * kHid;
-      for (int k = 0; k < kHid; ++k) act[k] = gbuf[c][k];
-    }
+    if (train)
+      store_act_rows(m.act, 2 * kHid, base + 32 * t, n, [&](int r, int c4) {
+        return make_float4(gbuf[r][c4], gbuf[r][c4 + 1], gbuf[r][c4 + 2], gbuf[r][c4 + 3]);
+      });
     // 3. o = W4 G + b4 (FP32): lane (c, g) computes output g of particle i
     float o = prm[kOffB4 + g];
     for (int k = 0; k < kHid; ++k) o = __builtin_fmaf(prm[kOffW4 + g * kHid + k], gbuf[c][k], o);
@@ -617,7 +628,8 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
 // are written in FP32 for the weight gradients and the pair backward.
 constexpr int kHeadRows = 32;
 __global__ __launch_bounds__(64) void k_sur_back_head(int n, SurrogateBufs m, float invN) {
-  __shared__ __align__(16) uint16_t tb[kHeadRows][kHbStride];  // dM of the tile, BF16, for the A operand
+  __shared__ __align__(16) uint16_t tb[kHeadRows][kHbStride];  // dG, then dM, of the tile, BF16 (A operands)
+  __shared__ float shD[kHeadRows][2];                          // dO of the tile's rows
   const int lane = threadIdx.x, c = lane & 31, g = lane >> 5;
   const int i = (int)blockIdx.x * kHeadRows + c;  // A-operand row c
   const bool live = i < n;
@@ -638,24 +650,47 @@ __global__ __launch_bounds__(64) void k_sur_back_head(int n, SurrogateBufs m, fl
       m.dO[2 * (size_t)i + 1] = d1;
     }
   }
-  // dG for this lane's K indices k = 16s + 8g + u of row c
-  bf16x8 ag[4];
+  if (g == 0) {
+    shD[c][0] = d0;
+    shD[c][1] = d1;
+  }
+  __syncthreads();
+  // dG = (dO W4) . [G > 0], row by row: 16 lanes take one row's 64 columns as
+  // float4 (columns c4 .. c4 + 3 in every row this lane visits), so the G
+  // reads and dG writes are 256 contiguous bytes per row; the BF16 copy in tb
+  // is the A operand of dM = dG W3.
   {
-    const float *G = m.act + (size_t)(live ? i : 0) * kActStride + 2 * kHid;
-    float *dG = m.dG + (size_t)(live ? i : 0) * kHid;
+    const int c4 = (lane & 15) * 4;
+    float wa[4], wb[4];
 #pragma unroll
-    for (int st = 0; st < 4; ++st) {
-      float v[8];
+    for (int j = 0; j < 4; ++j) {
+      wa[j] = prm[kOffW4 + c4 + j];
+      wb[j] = prm[kOffW4 + kHid + c4 + j];
+    }
+    for (int q = lane; q < kHeadRows * 16; q += 64) {
+      const int r = q >> 4, pi = (int)blockIdx.x * kHeadRows + r;
+      float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (pi < n) {
+        const float4 G4 = *reinterpret_cast<const float4 *>(m.act + (size_t)pi * kActStride + 2 * kHid + c4);
+        const float gv[4] = {G4.x, G4.y, G4.z, G4.w};
+        const float e0 = shD[r][0], e1 = shD[r][1];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int k = 16 * st + 8 * g + u;
-        const float x = __builtin_fmaf(prm[kOffW4 + k], d0, prm[kOffW4 + kHid + k] * d1);
-        v[u] = (live && G[k] > 0.0f) ? x : 0.0f;
-        if (live) dG[k] = v[u];
+        for (int j = 0; j < 4; ++j) {
+          const float x = __builtin_fmaf(wa[j], e0, wb[j] * e1);
+          v[j] = gv[j] > 0.0f ? x : 0.0f;
+        }
+        *reinterpret_cast<float4 *>(m.dG + (size_t)pi * kHid + c4) = make_float4(v[0], v[1], v[2], v[3]);
       }
-      ag[st] = bf8(pack2(v[0], v[1]), pack2(v[2], v[3]), pack2(v[4], v[5]), pack2(v[6], v[7]));
+      *reinterpret_cast<uint32_t *>(&tb[r][c4]) = pack2(v[0], v[1]);
+      *reinterpret_cast<uint32_t *>(&tb[r][c4 + 2]) = pack2(v[2], v[3]);
     }
   }
+  __syncthreads();
+  // A operands of dM: this lane's K indices k = 16s + 8g + u of row c
+  bf16x8 ag[4];
+#pragma unroll
+  for (int st = 0; st < 4; ++st) ag[st] = lds_bf8(&tb[c][16 * st + 8 * g]);
+  __syncthreads();  // tb is rewritten with dM below
   // dM = dG W3: B[k][n] = W3[k][n] = W3t[n][k]
   floatx16 mlo = zero16(), mhi = zero16();
 #pragma unroll
