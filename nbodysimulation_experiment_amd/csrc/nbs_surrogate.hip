@@ -777,6 +777,7 @@ __global__ __launch_bounds__(kLossBlock) void k_loss_final(const float *part, in
 // operand holds A[i][a] of row a = c (+32) and the B operand B[i][b] of
 // column b = c (+32), 32-lane coalesced row reads.
 constexpr int kOuterWaves = 2048;
+template <bool SCALE>
 __global__ __launch_bounds__(64) void k_outer_partial(int n, const float *A, int lda, int na, const float *B, int ldb,
                                                       const float *scale, float *part) {
   const int c = threadIdx.x & 31, g = threadIdx.x >> 5;
@@ -788,21 +789,34 @@ __global__ __launch_bounds__(64) void k_outer_partial(int n, const float *A, int
     for (int y = 0; y < 2; ++y) acc[x][y] = zero16();
   float bias[2] = {0.0f, 0.0f};
   for (int k0 = (int)blockIdx.x * 16; k0 < n; k0 += (int)gridDim.x * 16) {
+    // every load reads an in-range address (row and column clamped) and
+    // out-of-range values are zeroed afterwards: with no branch around them,
+    // the step's loads are all in flight together
+    float va[2][8], vb[2][8], sc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = min(k0 + 8 * g + u, n - 1);
+      sc[u] = SCALE ? scale[i] : 1.0f;
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        va[x][u] = A[(size_t)i * lda + min(32 * x + c, na - 1)];
+        vb[x][u] = B[(size_t)i * ldb + 32 * x + c];
+      }
+    }
     bf16x8 ao[2], bo[2];
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
-      float va[8], vb[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
-        const int i = k0 + 8 * g + u;
-        const bool in = i < n;
-        const int a = 32 * x + c;
-        va[u] = (in && a < na) ? A[(size_t)i * lda + a] : 0.0f;
-        vb[u] = in ? B[(size_t)i * ldb + 32 * x + c] : 0.0f;
-        if (in) bias[x] = __builtin_fmaf(va[u], scale ? scale[i] : 1.0f, bias[x]);
+        const bool in = k0 + 8 * g + u < n;
+        va[x][u] = (in && 32 * x + c < na) ? va[x][u] : 0.0f;
+        vb[x][u] = in ? vb[x][u] : 0.0f;
+        if (in) bias[x] = __builtin_fmaf(va[x][u], sc[u], bias[x]);
       }
-      ao[x] = bf8(pack2(va[0], va[1]), pack2(va[2], va[3]), pack2(va[4], va[5]), pack2(va[6], va[7]));
-      bo[x] = bf8(pack2(vb[0], vb[1]), pack2(vb[2], vb[3]), pack2(vb[4], vb[5]), pack2(vb[6], vb[7]));
+      ao[x] = bf8(pack2(va[x][0], va[x][1]), pack2(va[x][2], va[x][3]), pack2(va[x][4], va[x][5]),
+                  pack2(va[x][6], va[x][7]));
+      bo[x] = bf8(pack2(vb[x][0], vb[x][1]), pack2(vb[x][2], vb[x][3]), pack2(vb[x][4], vb[x][5]),
+                  pack2(vb[x][6], vb[x][7]));
     }
 #pragma unroll
     for (int x = 0; x < 2; ++x) {
@@ -833,6 +847,13 @@ __global__ __launch_bounds__(kBlock) void k_sum_parts(const float *part, int npa
   if (e >= width) return;
   float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   int q = 0;
+  for (; q + 32 <= nparts; q += 32) {  // 32 loads in flight, added in the order of the loop below
+    float v[32];
+#pragma unroll
+    for (int u = 0; u < 32; ++u) v[u] = part[(size_t)(q + u) * width + e];
+#pragma unroll
+    for (int u = 0; u < 32; ++u) acc[u & 7] += v[u];
+  }
   for (; q + 8 <= nparts; q += 8) {
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc[u] += part[(size_t)(q + u) * width + e];
@@ -1150,7 +1171,10 @@ static void outer_grad(int n, const float *A, int lda, int na, const float *B, i
   const unsigned blocks = outer_blocks(n);
   const int width = na * nb + na;
   float *sum = part + (size_t)blocks * width;
-  hipLaunchKernelGGL(k_outer_partial, dim3(blocks), dim3(64), 0, s, n, A, lda, na, B, ldb, scale, part);
+  if (scale)
+    hipLaunchKernelGGL(k_outer_partial<true>, dim3(blocks), dim3(64), 0, s, n, A, lda, na, B, ldb, scale, part);
+  else
+    hipLaunchKernelGGL(k_outer_partial<false>, dim3(blocks), dim3(64), 0, s, n, A, lda, na, B, ldb, scale, part);
   hipLaunchKernelGGL(k_sum_parts, dim3((width + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (const float *)part,
                      (int)blocks, width, sum);
   hipLaunchKernelGGL(k_scatter_grad, dim3((width + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (const float *)sum, na,
