@@ -397,28 +397,50 @@ template <int CAP>
 __device__ __forceinline__ void plan_tile(const uint32_t *start, const DevScene &sc, uint32_t kA, uint32_t kB,
                                           TilePlan &t) {
   const int64_t first = sc.cellBase, last = (int64_t)sc.cellBase + (int64_t)sc.nx * sc.ny - 1;
-  uint32_t total = 0;
+  // all six cell starts are loaded together (an empty range reads the first
+  // cell and is discarded), so the plan costs one memory round trip
+  uint32_t a[3], e[3];
+  bool ok[3];
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
     const int64_t lo = max((int64_t)kA + (int64_t)(d - 1) * sc.nx - 1, first);
     const int64_t hi = min((int64_t)kB + (int64_t)(d - 1) * sc.nx + 1, last);
-    uint32_t a = 0, e = 0;
-    if (lo <= hi) {
-      a = start[lo];
-      e = start[hi + 1];
-    }
-    t.rowStart[d] = a;
+    ok[d] = lo <= hi;
+    a[d] = start[ok[d] ? lo : first];
+    e[d] = start[(ok[d] ? hi : first) + 1];
+  }
+  uint32_t total = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const uint32_t a0 = ok[d] ? a[d] : 0u, e0 = ok[d] ? e[d] : 0u;
+    t.rowStart[d] = a0;
     t.off[d] = total;
-    total += e - a;
+    total += e0 - a0;
   }
   t.off[3] = total;
   t.use = (CAP > 0 && total <= (uint32_t)CAP) ? 1 : 0;
 }
 
 // Cooperative, coalesced copy of the planned ranges of src into the tile.
+// Each thread keeps four loads in flight.  Tile entry q of range d is slot
+// q + (rowStart[d] - off[d]), formed in 32 bits (the difference wraps).
+// Entries past the end are clamped to the last one, which is then loaded and
+// stored more than once with the same value.
 __device__ __forceinline__ void stage(const TilePlan &t, const float2 *__restrict__ src, float2 *dst) {
-  for (uint32_t q = threadIdx.x; q < t.off[3]; q += blockDim.x) {
-    const int d = (q >= t.off[1] ? 1 : 0) + (q >= t.off[2] ? 1 : 0);
-    dst[q] = src[t.rowStart[d] + (q - t.off[d])];
+  constexpr uint32_t kU = 4;
+  const uint32_t total = t.off[3], off1 = t.off[1], off2 = t.off[2];
+  const uint32_t b0 = t.rowStart[0] - t.off[0], b1 = t.rowStart[1] - off1, b2 = t.rowStart[2] - off2;
+  for (uint32_t q0 = threadIdx.x; q0 < total; q0 += kU * blockDim.x) {
+    float2 v[kU];
+    uint32_t at[kU];
+#pragma unroll
+    for (uint32_t u = 0; u < kU; ++u) {
+      at[u] = min(q0 + u * blockDim.x, total - 1u);
+      const uint32_t b = at[u] >= off2 ? b2 : (at[u] >= off1 ? b1 : b0);
+      v[u] = src[(uint32_t)(at[u] + b)];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kU; ++u) dst[at[u]] = v[u];
   }
 }
 
