@@ -380,6 +380,21 @@ __device__ __forceinline__ RowRange row_range(const uint32_t *start, const DevSc
   return {start[row + xlo], start[row + xhi + 1]};
 }
 
+// The three row ranges dy = -1, 0, 1 with their six loads in flight together:
+// a row outside the grid reads the particle's own row and is emptied after.
+__device__ __forceinline__ void row_ranges(const uint32_t *start, const DevScene &sc, int cx, int cy, RowRange rr[3]) {
+  const int xlo = max(cx - 1, 0), xhi = min(cx + 1, sc.nx - 1);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int yy = cy + d - 1;
+    const bool in = yy >= 0 && yy < sc.ny;
+    const uint32_t row = sc.cellBase + (uint32_t)((in ? yy : cy) * sc.nx);
+    rr[d].j0 = start[row + xlo];
+    rr[d].j1 = start[row + xhi + 1];
+    if (!in) rr[d].j1 = rr[d].j0;
+  }
+}
+
 // LDS tile of a workgroup (positions only: every candidate's position is read
 // by the cheap pass, the other per-particle data only for in-kernel pairs, from
 // the slot arrays).  The workgroup's particles occupy consecutive

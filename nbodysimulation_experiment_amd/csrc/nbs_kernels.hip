@@ -44,8 +44,10 @@ __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const D
                                                   const float2 *X, Idx idx, float2 xi, float h2, Term &&term,
                                                   Apply &&apply) {
   uint32_t cand = 0;
+  RowRange rows[3];
+  row_ranges(start, sc, cx, cy, rows);
   for (int d = 0; d < 3; ++d) {
-    const RowRange rr = row_range(start, sc, cx, cy, d - 1);
+    const RowRange rr = rows[d];
     cand += rr.j1 - rr.j0;
     for (uint32_t base = rr.j0; base < rr.j1; base += 32u) {
       const uint32_t cnt = min(32u, rr.j1 - base);
