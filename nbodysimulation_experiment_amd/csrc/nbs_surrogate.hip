@@ -310,24 +310,23 @@ __device__ __forceinline__ uint32_t chunk_bits(const float2 *xs, uint32_t cnt, f
 constexpr uint32_t kWalkChunk = 16;
 constexpr uint32_t kWalkRowMax = 32;
 
-__device__ __forceinline__ bool rows_fit(const Buffers &b, const PartInfo &q, bool live) {
+// rows: the lane's three candidate rows (row_ranges; ignored for a lane past
+// n).
+__device__ __forceinline__ bool rows_fit(const RowRange (&rows)[3], bool live) {
   bool fit = true;
   if (live)
-    for (int dy = -1; dy <= 1; ++dy) {
-      const RowRange rr = row_range(b.startNext, q.sc, q.cx, q.cy, dy);
-      fit = fit && rr.j1 - rr.j0 <= kWalkRowMax;
-    }
+    for (int d = 0; d < 3; ++d) fit = fit && rows[d].j1 - rows[d].j0 <= kWalkRowMax;
   return __ballot(!fit) == 0ull;
 }
 
 // Tests the lane's three rows (every row at most kWalkRowMax candidates) and
 // loads their masks into the cursor; the particle itself is no pair.
-__device__ __forceinline__ void cursor_rows(Cursor &cu, const Buffers &b, const DevParams &p, const PairSrc &src,
-                                            const PartInfo &q, int i) {
+__device__ __forceinline__ void cursor_rows(Cursor &cu, const RowRange (&rows)[3], const DevParams &p,
+                                            const PairSrc &src, const PartInfo &q, int i) {
   uint32_t m[3], base[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-    const RowRange rr = row_range(b.startNext, q.sc, q.cx, q.cy, r - 1);
+    const RowRange rr = rows[r];
     const uint32_t cnt = rr.j1 - rr.j0;
     // the row's first index in the source: j0 + base(r) wraps modulo 2^32
     // (off - rowStart on the LDS tile), so it is formed in 32 bits first
@@ -483,8 +482,13 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
   auto pair_pass = [&](const PairSrc &sp) {
     // rows up front except for the multi-scene training forward, where they
     // measured slower
-    const bool rows = !(MULTI && (mode & kFwdTrain)) && rows_fit(b, q, me < n);
-    if (rows && me < n) cursor_rows(cu, b, p, sp, q, me);
+    bool rows = false;
+    if (!(MULTI && (mode & kFwdTrain))) {
+      RowRange rr[3] = {};
+      if (me < n) row_ranges(b.startNext, q.sc, q.cx, q.cy, rr);  // the six cell starts in flight together
+      rows = rows_fit(rr, me < n);
+      if (rows && me < n) cursor_rows(cu, rr, p, sp, q, me);
+    }
     for (;;) {
       uint32_t t = 0;
       float2 xj = make_float2(0.0f, 0.0f);
