@@ -520,6 +520,21 @@ __device__ __forceinline__ uint32_t rank_in_cell(Slots slots, uint32_t c0, uint3
   return rank;
 }
 
+// Moves particle s of the previous slot order to slot d of the new one, key
+// k.  Every field is loaded before the first store: the destination arrays
+// are distinct from the sources, and loading first keeps the loads in flight
+// together (interleaved, each load would wait for the store before it).
+__device__ __forceinline__ void place_move(const Buffers &b, uint32_t s, uint32_t d, uint32_t k) {
+  const float2 xp = b.xp[s], x0 = b.x0[s];
+  const uint32_t id = b.id[s];
+  const float2 dn = b.densTo ? b.densFrom[s] : make_float2(0.0f, 0.0f);
+  b.x1[d] = xp;
+  b.prev[d] = x0;  // demo4.cpp:334 prevPosition = curPosition
+  b.id2[d] = id;
+  b.keys[d] = k;
+  if (b.densTo) b.densTo[d] = dn;
+}
+
 __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
   __shared__ uint32_t tileSlots[kRankTile];
   __shared__ uint32_t span[2];
@@ -544,12 +559,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
   if (!live) return;
   const uint32_t rank = tiled ? rank_in_cell([&](uint32_t q) { return tileSlots[q - lo]; }, c0, c1, s)
                               : rank_in_cell([&](uint32_t q) { return b.tmpSlot[q]; }, c0, c1, s);
-  const uint32_t d = c0 + rank;
-  b.x1[d] = b.xp[s];
-  b.prev[d] = b.x0[s];  // demo4.cpp:334 prevPosition = curPosition
-  b.id2[d] = b.id[s];
-  b.keys[d] = key;
-  if (b.densTo) b.densTo[d] = b.densFrom[s];
+  place_move(b, s, c0 + rank, key);
 }
 
 // ---------------------------------------------------------------------------
@@ -614,14 +624,6 @@ __device__ __forceinline__ void for_each_place_source(const Buffers &b, const De
     const uint32_t k = b.keyp[s] & ~kFarFlag;
     if (k >= cf && k <= cl) f(s, k);
   }
-}
-
-__device__ __forceinline__ void place_move(const Buffers &b, uint32_t s, uint32_t d, uint32_t k) {
-  b.x1[d] = b.xp[s];
-  b.prev[d] = b.x0[s];  // demo4.cpp:334 prevPosition = curPosition
-  b.id2[d] = b.id[s];
-  b.keys[d] = k;
-  if (b.densTo) b.densTo[d] = b.densFrom[s];
 }
 
 template <bool MULTI>
