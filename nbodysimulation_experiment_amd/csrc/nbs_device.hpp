@@ -306,9 +306,14 @@ __device__ __forceinline__ uint64_t body_candidates(const DevBody *bodies, int c
   const int t = threadIdx.x & 63;
   bool may = false;
   if (t < cnt) {
+    // the type and the whole cull box (16 bytes at offset 32 of the 48-byte
+    // record) are loaded together and tested without short-circuiting, so the
+    // test costs one memory round trip
     const DevBody &bd = bodies[c0 + t];
-    may = bd.type != BODY_CIRCLE ||
-          !(bd.cull[0] > box.x1 || bd.cull[2] < box.x0 || bd.cull[1] > box.y1 || bd.cull[3] < box.y0);
+    const int type = bd.type;
+    const int4 cl = *reinterpret_cast<const int4 *>(bd.cull);
+    const bool outside = (cl.x > box.x1) | (cl.z < box.x0) | (cl.y > box.y1) | (cl.w < box.y0);
+    may = (type != BODY_CIRCLE) | !outside;
   }
   return __ballot(may);
 }

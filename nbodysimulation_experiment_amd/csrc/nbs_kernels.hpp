@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "../../include/nbs_sph.h"
@@ -50,6 +51,8 @@ struct DevBody {
   // the circle's collision test (x0, y0, x1, y1); full range for other bodies.
   int cull[4];
 };
+// body_candidates reads cull as one 16-byte load
+static_assert(sizeof(DevBody) == 48 && offsetof(DevBody, cull) == 32, "DevBody layout");
 
 // Order-preserving integer image of a float: for non-NaN a, b,
 // a < b implies float_order(a) < float_order(b) (and -0 maps below +0).
