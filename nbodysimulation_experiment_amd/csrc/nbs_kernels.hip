@@ -535,6 +535,32 @@ __device__ __forceinline__ void place_move(const Buffers &b, uint32_t s, uint32_
   if (b.densTo) b.densTo[d] = dn;
 }
 
+// K moves at once: every load of the K particles is issued before the first
+// store.  Move u goes from slot s[u] to slot d[u] with key k[u]; d[u] =
+// 0xFFFFFFFF moves nothing (its loads read slot s[u] = 0, always in range).
+template <uint32_t K>
+__device__ __forceinline__ void place_moves(const Buffers &b, const uint32_t (&s)[K], const uint32_t (&d)[K],
+                                            const uint32_t (&k)[K]) {
+  float2 xp[K], x0[K], dn[K];
+  uint32_t id[K];
+#pragma unroll
+  for (uint32_t u = 0; u < K; ++u) {
+    xp[u] = b.xp[s[u]];
+    x0[u] = b.x0[s[u]];
+    id[u] = b.id[s[u]];
+    dn[u] = b.densTo ? b.densFrom[s[u]] : make_float2(0.0f, 0.0f);
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < K; ++u) {
+    if (d[u] == 0xFFFFFFFFu) continue;
+    b.x1[d[u]] = xp[u];
+    b.prev[d[u]] = x0[u];  // demo4.cpp:334 prevPosition = curPosition
+    b.id2[d[u]] = id[u];
+    b.keys[d[u]] = k[u];
+    if (b.densTo) b.densTo[d[u]] = dn[u];
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
   __shared__ uint32_t tileSlots[kRankTile];
   __shared__ uint32_t span[2];
@@ -580,44 +606,75 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
 // cells of thousands of particles).
 // ---------------------------------------------------------------------------
 
+// Slot ranges, in the last binning, of the rows below, at and above the cells
+// [cf, cl] of scene sc (one cell wider on each side), merged in key order.
+// The six bounds are loaded together; absent windows are empty.
+struct PlaceWindows {
+  uint32_t ws[3], we[3];
+};
+
+__device__ __forceinline__ PlaceWindows place_windows(const Buffers &b, const DevScene &sc, uint32_t cf,
+                                                      uint32_t cl) {
+  const int64_t base = sc.cellBase, last = base + (int64_t)sc.nx * sc.ny - 1;
+  const int64_t a = max((int64_t)cf, base), e = min((int64_t)cl, last);
+  int64_t wa[3] = {a, a, a}, wb[3] = {e, e, e};
+  int m = 0;
+  for (int dy = -1; dy <= 1; ++dy) {
+    const int64_t lo = max(a + (int64_t)dy * sc.nx - 1, base), hi = min(e + (int64_t)dy * sc.nx + 1, last);
+    if (lo > hi) continue;
+    if (m > 0 && lo <= wb[m - 1] + 1) {
+      wb[m - 1] = max(wb[m - 1], hi);
+    } else {
+      wa[m] = lo;
+      wb[m] = hi;
+      ++m;
+    }
+  }
+  PlaceWindows pw;
+#pragma unroll
+  for (int w = 0; w < 3; ++w) {
+    pw.ws[w] = b.startCur[wa[w]];
+    pw.we[w] = b.startCur[wb[w] + 1];
+  }
+#pragma unroll
+  for (int w = 0; w < 3; ++w)
+    if (w >= m) pw.we[w] = pw.ws[w];
+  return pw;
+}
+
+// Calls f(s, key) for every slot s of the windows whose key is in [cf, cl]
+// (far movers carry the flag and fall outside).  The windows are walked as one
+// range, each thread taking a strided share with kU key loads in flight.
+template <uint32_t kU, typename F>
+__device__ __forceinline__ void for_each_window_source(const Buffers &b, const PlaceWindows &pw, uint32_t cf,
+                                                       uint32_t cl, F &&f) {
+  const uint32_t o1 = pw.we[0] - pw.ws[0], o2 = o1 + (pw.we[1] - pw.ws[1]), total = o2 + (pw.we[2] - pw.ws[2]);
+  for (uint32_t q0 = threadIdx.x; q0 < total; q0 += kU * kBlock) {
+    uint32_t sl[kU], k[kU];
+#pragma unroll
+    for (uint32_t u = 0; u < kU; ++u) {
+      const uint32_t at = min(q0 + u * kBlock, total - 1u);
+      sl[u] = at >= o2 ? pw.ws[2] + (at - o2) : (at >= o1 ? pw.ws[1] + (at - o1) : pw.ws[0] + at);
+      k[u] = b.keyp[sl[u]];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kU; ++u)
+      if (q0 + u * kBlock < total && k[u] >= cf && k[u] <= cl) f(sl[u], k[u]);
+  }
+}
+
 // Calls f(s, key) for every particle whose new cell key is in [cf, cl]; each
 // thread takes a strided share.  Window slots of far movers are skipped (they
-// come from the far list).
-template <bool MULTI, typename F>
-__device__ __forceinline__ void for_each_place_source(const Buffers &b, const DevScene &s0, uint32_t cf, uint32_t cl,
-                                                      uint32_t farN, F &&f) {
-  const int sFirst = MULTI ? scene_of_key(b.sceneBase, b.nScenes, cf) : 0;
-  const int sLast = MULTI ? scene_of_key(b.sceneBase, b.nScenes, cl) : 0;
-  for (int si = sFirst; si <= sLast; ++si) {
-    const DevScene sc = MULTI ? b.scenes[si] : s0;
-    const int64_t base = sc.cellBase, last = base + (int64_t)sc.nx * sc.ny - 1;
-    const int64_t a = max((int64_t)cf, base), e = min((int64_t)cl, last);
-    // key windows of the rows below, at and above [a, e], merged in key order
-    int64_t wa[3], wb[3];
-    int m = 0;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int64_t lo = max(a + (int64_t)dy * sc.nx - 1, base), hi = min(e + (int64_t)dy * sc.nx + 1, last);
-      if (lo > hi) continue;
-      if (m > 0 && lo <= wb[m - 1] + 1) {
-        wb[m - 1] = max(wb[m - 1], hi);
-      } else {
-        wa[m] = lo;
-        wb[m] = hi;
-        ++m;
-      }
-    }
-    for (int w = 0; w < m; ++w) {
-      const uint32_t s1 = b.startCur[wb[w] + 1];
-      // four independent key loads in flight per thread
-      for (uint32_t s = b.startCur[wa[w]] + threadIdx.x; s < s1; s += 4u * kBlock) {
-        uint32_t k[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) k[u] = (s + u * kBlock < s1) ? b.keyp[s + u * kBlock] : 0xFFFFFFFFu;
-#pragma unroll
-        for (int u = 0; u < 4; ++u)  // far movers carry the flag and fall outside [cf, cl]
-          if (k[u] >= cf && k[u] <= cl) f(s + u * kBlock, k[u]);
-      }
-    }
+// come from the far list).  pw0 holds the windows of a one-scene launch.
+template <bool MULTI, uint32_t kU, typename F>
+__device__ __forceinline__ void for_each_place_source(const Buffers &b, const PlaceWindows &pw0, uint32_t cf,
+                                                      uint32_t cl, uint32_t farN, F &&f) {
+  if (MULTI) {
+    const int sFirst = scene_of_key(b.sceneBase, b.nScenes, cf), sLast = scene_of_key(b.sceneBase, b.nScenes, cl);
+    for (int si = sFirst; si <= sLast; ++si)
+      for_each_window_source<kU>(b, place_windows(b, b.scenes[si], cf, cl), cf, cl, f);
+  } else {
+    for_each_window_source<kU>(b, pw0, cf, cl, f);
   }
   for (uint32_t q = threadIdx.x; q < farN; q += kBlock) {
     const uint32_t s = b.farList[q];
@@ -632,24 +689,24 @@ __global__ __launch_bounds__(kBlock) void k_place(Buffers b, DevScene s0, int n,
   __shared__ uint32_t bkey[kPlaceCap];     // its new cell
   __shared__ uint32_t fillc[kPlaceCap];    // arrivals per cell, at the cell's first bucket entry
   __shared__ uint32_t cstart[kPlaceCap + 1];  // new cell starts of [cf, cl + 1]
-  __shared__ uint32_t span[2];
   const uint32_t g0 = blockIdx.x * kPlaceChunk, g1 = min(g0 + kPlaceChunk, (uint32_t)n);
-  if (threadIdx.x < 2) span[threadIdx.x] = b.chunkCells[2 * blockIdx.x + threadIdx.x];
-  __syncthreads();
-  const uint32_t cf = span[0], cl = span[1];
+  const uint32_t cf = b.chunkCells[2 * blockIdx.x], cl = b.chunkCells[2 * blockIdx.x + 1];
+  // every bound the workgroup needs is requested before the first is used
+  PlaceWindows pw0{};
+  if (!MULTI) pw0 = place_windows(b, s0, cf, cl);
   const uint32_t lo = b.startNext[cf], hi = b.startNext[cl + 1];
   const uint32_t farN = min(*b.farCount, (uint32_t)n);
+  const bool staged = cl - cf + 2u <= kPlaceCap + 1u;
+  if (staged)
+    for (uint32_t q = threadIdx.x; q < cl - cf + 2u; q += kBlock) cstart[q] = b.startNext[cf + q];
   if (hi - lo <= kPlaceCap) {
-    const bool staged = cl - cf + 2u <= kPlaceCap + 1u;
     for (uint32_t q = threadIdx.x; q < hi - lo; q += kBlock) {
       fillc[q] = 0u;
       bucket[q] = 0xFFFFFFFFu;
     }
-    if (staged)
-      for (uint32_t q = threadIdx.x; q < cl - cf + 2u; q += kBlock) cstart[q] = b.startNext[cf + q];
     __syncthreads();
     auto start_of = [&](uint32_t k) { return staged ? cstart[k - cf] : b.startNext[k]; };
-    for_each_place_source<MULTI>(b, s0, cf, cl, farN, [&](uint32_t s, uint32_t k) {
+    for_each_place_source<MULTI, 16>(b, pw0, cf, cl, farN, [&](uint32_t s, uint32_t k) {
       const uint32_t c0 = start_of(k), count = start_of(k + 1) - c0;
       const uint32_t a = atomicAdd(&fillc[c0 - lo], 1u);  // LDS histogram: arrival in the cell's bucket
       if (a < count) {
@@ -658,19 +715,35 @@ __global__ __launch_bounds__(kBlock) void k_place(Buffers b, DevScene s0, int n,
       }
     });
     __syncthreads();
-    for (uint32_t q = threadIdx.x; q < hi - lo; q += kBlock) {
-      const uint32_t s = bucket[q];
-      if (s >= (uint32_t)n) continue;  // never, when every particle is found once
-      const uint32_t k = bkey[q];
-      const uint32_t c0 = start_of(k), c1 = start_of(k + 1);
-      if (c1 <= g0 || c0 >= g1) continue;  // a cell the neighbouring workgroup moves
-      const uint32_t d = c0 + rank_in_cell([&](uint32_t e) { return bucket[e - lo]; }, c0, c1, s);
-      if (d >= g0 && d < g1) place_move(b, s, d, k);
+    // kMoves moves per thread at a time: their loads are issued before their
+    // stores (a slot that moves nothing loads slot 0 and stores nothing)
+    constexpr uint32_t kMoves = 4;
+    for (uint32_t q0 = threadIdx.x; q0 < hi - lo; q0 += kMoves * kBlock) {
+      uint32_t ms[kMoves], md[kMoves], mk[kMoves];
+#pragma unroll
+      for (uint32_t u = 0; u < kMoves; ++u) {
+        const uint32_t q = q0 + u * kBlock;
+        ms[u] = 0u;
+        md[u] = 0xFFFFFFFFu;  // no move
+        mk[u] = 0u;
+        if (q >= hi - lo) continue;
+        const uint32_t s = bucket[q];
+        if (s >= (uint32_t)n) continue;  // never, when every particle is found once
+        const uint32_t k = bkey[q];
+        const uint32_t c0 = start_of(k), c1 = start_of(k + 1);
+        if (c1 <= g0 || c0 >= g1) continue;  // a cell the neighbouring workgroup moves
+        const uint32_t d = c0 + rank_in_cell([&](uint32_t e) { return bucket[e - lo]; }, c0, c1, s);
+        if (d < g0 || d >= g1) continue;
+        ms[u] = s;
+        md[u] = d;
+        mk[u] = k;
+      }
+      place_moves<kMoves>(b, ms, md, mk);
     }
     return;
   }
   // Oversized cells: rank each source against every source of its cell.
-  for_each_place_source<MULTI>(b, s0, cf, cl, farN, [&](uint32_t s, uint32_t k) {
+  for_each_place_source<MULTI, 1>(b, pw0, cf, cl, farN, [&](uint32_t s, uint32_t k) {
     const uint32_t c0 = b.startNext[k], c1 = b.startNext[k + 1];
     if (c1 <= g0 || c0 >= g1) return;
     uint32_t rank = 0;
