@@ -6,7 +6,8 @@
 //                       the per-cell arrival count (demo4.cpp:301-356)
 //   2. k_scan_reduce + k_scan_final (k_scan_small for small grids)
 //                       exclusive scan of the per-cell counts -> cell starts
-//   3. k_scatter + k_rank_reorder
+//   3. k_place (grids above kSmallGridCells cells once every particle is
+//      binned), else k_scatter + k_rank_reorder
 //                       bucket the particles by cell, order each cell by the
 //                       particles' previous slots (a stable counting sort of the
 //                       slot order), and move the state into the new slot order
@@ -16,7 +17,8 @@
 //                       (demo4.cpp:239-255, 411-450)
 // Small steps use the wide variants of 1, 4 and 5 (one wave per particle;
 // 1 is then followed by k_count_fill, or for the smallest steps the counting
-// moves into the one-workgroup scan, k_count_scan_small).
+// and the bucketing move into the one-workgroup scan, k_count_scan_small,
+// whose records k_rank_reorder reads).
 //
 // Every FP32 expression keeps the reference's operand order and the library
 // is compiled with -ffp-contract=off and correctly rounded sqrt / divide, so the
@@ -561,6 +563,9 @@ __device__ __forceinline__ void place_moves(const Buffers &b, const uint32_t (&s
   }
 }
 
+// REC: the bucketing wrote a record per new slot (binRec), so each thread
+// starts from one load instead of three dependent ones.
+template <bool REC>
 __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
   __shared__ uint32_t tileSlots[kRankTile];
   __shared__ uint32_t span[2];
@@ -569,10 +574,15 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
   const bool live = p < n;
   uint32_t s = 0, key = 0, c0 = 0, c1 = 0;
   if (live) {
-    s = b.tmpSlot[p];
-    key = b.keyp[s];
-    c0 = b.startNext[key];
-    c1 = b.startNext[key + 1];
+    if (REC) {
+      const uint4 r = b.binRec[p];
+      s = r.x, key = r.y, c0 = r.z, c1 = r.w;
+    } else {
+      s = b.tmpSlot[p];
+      key = b.keyp[s];
+      c0 = b.startNext[key];
+      c1 = b.startNext[key + 1];
+    }
   }
   if (p == p0) span[0] = c0;
   if (p == min(p0 + kBlock, n) - 1) span[1] = c1;
@@ -772,8 +782,11 @@ __global__ __launch_bounds__(kBlock) void k_place(Buffers b, DevScene s0, int n,
 // (in cellStats[0..1]).
 // The one-workgroup scan body: counts from LDS (hist, when the workgroup has
 // just counted there) or from the global counters (reset for the next step).
+// With ldsStarts the cell starts [0, cells] also go to LDS (which may be hist:
+// every count is read before the first start is written).
 template <bool LDS_COUNTS>
-__device__ __forceinline__ void scan_small_body(Buffers &b, uint32_t cells, const uint32_t *hist) {
+__device__ __forceinline__ void scan_small_body(Buffers &b, uint32_t cells, const uint32_t *hist,
+                                                uint32_t *ldsStarts = nullptr) {
   constexpr int kWaves = kSmallGridBlock / 64;
   constexpr uint32_t kItems = kSmallGridCells / kSmallGridBlock;  // consecutive cells per thread
   __shared__ uint32_t waveTot[kWaves], waveMin[kWaves], waveMax[kWaves];
@@ -809,6 +822,7 @@ __device__ __forceinline__ void scan_small_body(Buffers &b, uint32_t cells, cons
   for (uint32_t k = 0; k < kItems; ++k) {
     if (c0 + k < cells) {
       b.startNext[c0 + k] = run;
+      if (ldsStarts) ldsStarts[c0 + k] = run;
       if (!LDS_COUNTS) b.count[c0 + k] = 0u;  // ready for the next step's counting
     }
     run += v[k];
@@ -818,6 +832,7 @@ __device__ __forceinline__ void scan_small_body(Buffers &b, uint32_t cells, cons
     b.cellStats[0] = mn;
     b.cellStats[1] = mx;
     b.startNext[cells] = tot;
+    if (ldsStarts) ldsStarts[cells] = tot;
   }
 }
 
@@ -825,20 +840,45 @@ __global__ __launch_bounds__(kSmallGridBlock) void k_scan_small(Buffers b, uint3
   scan_small_body<false>(b, cells, nullptr);
 }
 
-// Counting and scan of a small step in one workgroup: the cell counts are an
-// LDS histogram (kSmallGridCells counters), each particle's arrival index in
-// its cell is the LDS atomic's old value, then the scan runs from LDS.  Used
-// after the wide viscosity kernel (which leaves the counting to a later
-// kernel) for steps of at most kSmallCountParticles particles: one launch
-// instead of k_count_fill + k_scan_small, and no global atomics.
+// Counting, scan and scatter of a small step in one workgroup: the cell
+// counts are an LDS histogram (kSmallGridCells counters), each particle's
+// arrival index in its cell is the LDS atomic's old value, the scan runs from
+// LDS and leaves the cell starts there, and each particle is then bucketed at
+// its cell start + arrival index: its source slot in tmpSlot, and in binRec
+// the record (source slot, key, cell start, cell end) the rank pass starts
+// from.  Used after the wide viscosity kernel (which leaves the counting to a
+// later kernel) for steps of at most kSmallCountParticles particles: one
+// launch instead of k_count_fill + k_scan_small + k_scatter, and no global
+// atomics.
+constexpr int kSmallItems = kSmallCountParticles / kSmallGridBlock;  // particles per thread, at most
 __global__ __launch_bounds__(kSmallGridBlock) void k_count_scan_small(Buffers b, int n, uint32_t cells) {
-  __shared__ uint32_t hist[kSmallGridCells];
+  __shared__ uint32_t hist[kSmallGridCells + 1];  // counts, then cell starts [0, cells]
   for (uint32_t c = threadIdx.x; c < cells; c += kSmallGridBlock) hist[c] = 0u;
+  uint32_t key[kSmallItems], fill[kSmallItems];
+#pragma unroll
+  for (int u = 0; u < kSmallItems; ++u) {  // the key loads in flight together
+    if (u * kSmallGridBlock >= n) break;    // workgroup-uniform
+    key[u] = b.keyp[min((int)threadIdx.x + u * kSmallGridBlock, n - 1)];
+  }
   __syncthreads();
-  for (int i = (int)threadIdx.x; i < n; i += kSmallGridBlock)
-    b.fill[i] = atomicAdd(&hist[b.keyp[i]], 1u);  // arrival slot; the rank pass orders the cell
+#pragma unroll
+  for (int u = 0; u < kSmallItems; ++u) {
+    if (u * kSmallGridBlock >= n) break;
+    fill[u] = (int)threadIdx.x + u * kSmallGridBlock < n ? atomicAdd(&hist[key[u]], 1u) : 0u;  // arrival
+  }
   __syncthreads();
-  scan_small_body<true>(b, cells, hist);
+  scan_small_body<true>(b, cells, hist, hist);
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kSmallItems; ++u) {
+    if (u * kSmallGridBlock >= n) break;
+    const int i = (int)threadIdx.x + u * kSmallGridBlock;
+    if (i < n) {
+      const uint32_t c0 = hist[key[u]], c1 = hist[key[u] + 1], pos = c0 + fill[u];
+      b.tmpSlot[pos] = (uint32_t)i;
+      b.binRec[pos] = make_uint4((uint32_t)i, key[u], c0, c1);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1368,7 +1408,7 @@ void launch_scan(const Buffers &b, uint32_t cells, int n, hipStream_t s) {
 void launch_scatter_reorder(const Buffers &b, int n, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_scatter, dim3(grid_for(n)), dim3(kBlock), 0, s, b, n);
-  hipLaunchKernelGGL(k_rank_reorder, dim3(grid_for(n)), dim3(kBlock), 0, s, b, n);
+  hipLaunchKernelGGL(k_rank_reorder<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, b, n);
 }
 
 bool place_binning(uint32_t cells, int viscCap, bool allBinned) {
@@ -1390,7 +1430,10 @@ uint32_t launch_grid(const Buffers &b, const DevScene &s0, uint32_t cells, int n
     return (cells + kScanChunk - 1) / kScanChunk;
   }
   if (count_in_grid(n, cells, viscCap)) {
+    // counted, scanned and bucketed by one workgroup; the rank pass reads its records
     hipLaunchKernelGGL(k_count_scan_small, dim3(1), dim3(kSmallGridBlock), 0, s, b, n, cells);
+    if (n > 0) hipLaunchKernelGGL(k_rank_reorder<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, b, n);
+    return slots;
   } else if (cells <= kSmallGridCells) {
     hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kSmallGridBlock), 0, s, b, cells);
   } else {

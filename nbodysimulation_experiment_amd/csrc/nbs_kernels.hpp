@@ -77,6 +77,7 @@ struct Buffers {
   uint32_t *keyp;      // cell key of the predicted position (previous slot order)
   uint32_t *fill;      // arrival index within the new cell
   uint32_t *tmpSlot;   // source slots bucketed by new cell
+  uint4 *binRec;       // per new slot of a one-workgroup binning: source slot, key, cell start, cell end
   uint32_t *scn;       // scene of particles not yet binned
   uint32_t *count;     // per cell, zero between steps
   uint32_t *startCur;  // per cell + 1: the last binning

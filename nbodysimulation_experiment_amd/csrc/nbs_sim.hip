@@ -720,7 +720,7 @@ class Simulation {
     grow2(d_.x0); grow2(d_.prev); grow2(d_.v); grow2(d_.dens); grow2(d_.pres); grow2(d_.acc);
     grow4(d_.id); grow4(d_.keys); grow4(d_.scn);
     d_.xp.alloc(cap); d_.x1.alloc(cap); d_.id2.alloc(cap); d_.keyp.alloc(cap); d_.fill.alloc(cap);
-    d_.tmpSlot.alloc(cap); d_.farList.alloc(cap);
+    d_.tmpSlot.alloc(cap); d_.binRec.alloc(cap); d_.farList.alloc(cap);
     d_.chunkCells.alloc(2 * ((cap + kPlaceChunk - 1) / kPlaceChunk));
     d_.candStats.alloc(2 * (size_t)density_stat_slots((int)cap, kWideCap));
     cap_ = cap;
@@ -767,7 +767,7 @@ class Simulation {
     buf_.x0 = d_.x0.p; buf_.xp = d_.xp.p; buf_.x1 = d_.x1.p; buf_.prev = d_.prev.p; buf_.v = d_.v.p;
     buf_.dens = d_.dens.p; buf_.pres = d_.pres.p; buf_.acc = d_.acc.p;
     buf_.id = d_.id.p; buf_.id2 = d_.id2.p; buf_.keys = d_.keys.p; buf_.keyp = d_.keyp.p; buf_.fill = d_.fill.p;
-    buf_.tmpSlot = d_.tmpSlot.p; buf_.scn = d_.scn.p;
+    buf_.tmpSlot = d_.tmpSlot.p; buf_.binRec = d_.binRec.p; buf_.scn = d_.scn.p;
     buf_.farList = nullptr; buf_.farCount = buf_.farCountClear = nullptr; buf_.chunkCells = nullptr;
     buf_.count = d_.count.p; buf_.startCur = d_.startCur.p; buf_.startNext = d_.startNext.p;
     buf_.blockSum = d_.blockSum.p;
@@ -1381,6 +1381,7 @@ class Simulation {
     DevBuf<uint32_t> id, id2, keys, keyp, fill, tmpSlot, scn, count, startCur, startNext, blockSum;
     DevBuf<uint32_t> farList, farCounts;  // placement binning: far movers, one counter per step parity
     DevBuf<uint32_t> chunkCells;          // placement binning: first and last cell of each chunk
+    DevBuf<uint4> binRec;                 // one-workgroup binning: a record per new slot for the rank pass
     DevBuf<uint32_t> candStats, cellStats;
   };
 
