@@ -783,20 +783,39 @@ __global__ __launch_bounds__(kBlock) void k_place(Buffers b, DevScene s0, int n,
 // The one-workgroup scan body: counts from LDS (hist, when the workgroup has
 // just counted there) or from the global counters (reset for the next step).
 // With ldsStarts the cell starts [0, cells] also go to LDS (which may be hist:
-// every count is read before the first start is written).
+// every count is read before the first start is written).  Thread t takes the
+// kItems consecutive cells from kItems * t: in LDS they sit at lds_cell(c),
+// one padding word per 32 cells, so that the 32 lanes of a half-wave reading
+// their k-th cell hit 32 different banks; in global memory a thread's cells are
+// moved as 16-byte vectors.
+__device__ __forceinline__ uint32_t lds_cell(uint32_t c) { return c + (c >> 5); }
+constexpr uint32_t kSmallHistWords = kSmallGridCells + 1 + ((kSmallGridCells + 1) >> 5) + 1;
+
 template <bool LDS_COUNTS>
 __device__ __forceinline__ void scan_small_body(Buffers &b, uint32_t cells, const uint32_t *hist,
                                                 uint32_t *ldsStarts = nullptr) {
   constexpr int kWaves = kSmallGridBlock / 64;
   constexpr uint32_t kItems = kSmallGridCells / kSmallGridBlock;  // consecutive cells per thread
+  static_assert(kItems % 4 == 0, "cells per thread in 16-byte vectors");
   __shared__ uint32_t waveTot[kWaves], waveMin[kWaves], waveMax[kWaves];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t c0 = threadIdx.x * kItems;
+  const bool whole = c0 + kItems <= cells;  // every cell of the thread exists
   uint32_t v[kItems];
+  if (!LDS_COUNTS && whole) {
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; k += 4) {
+      const uint4 q = *reinterpret_cast<const uint4 *>(b.count + c0 + k);
+      v[k] = q.x, v[k + 1] = q.y, v[k + 2] = q.z, v[k + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; ++k)
+      v[k] = (c0 + k < cells) ? (LDS_COUNTS ? hist[lds_cell(c0 + k)] : b.count[c0 + k]) : 0u;
+  }
   uint32_t sum = 0, mn = 0xFFFFFFFFu, mx = 0u;
 #pragma unroll
   for (uint32_t k = 0; k < kItems; ++k) {
-    v[k] = (c0 + k < cells) ? (LDS_COUNTS ? hist[c0 + k] : b.count[c0 + k]) : 0u;
     sum += v[k];
     mx = max(mx, v[k]);
     if (v[k] > 0u) mn = min(mn, v[k]);
@@ -818,21 +837,38 @@ __device__ __forceinline__ void scan_small_body(Buffers &b, uint32_t cells, cons
     tot += t;
   }
   run += incl - sum;
+  uint32_t st[kItems];  // exclusive starts of the thread's cells
 #pragma unroll
   for (uint32_t k = 0; k < kItems; ++k) {
-    if (c0 + k < cells) {
-      b.startNext[c0 + k] = run;
-      if (ldsStarts) ldsStarts[c0 + k] = run;
-      if (!LDS_COUNTS) b.count[c0 + k] = 0u;  // ready for the next step's counting
-    }
+    st[k] = run;
     run += v[k];
+  }
+  if (whole) {
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; k += 4) {
+      *reinterpret_cast<uint4 *>(b.startNext + c0 + k) = make_uint4(st[k], st[k + 1], st[k + 2], st[k + 3]);
+      if (!LDS_COUNTS) *reinterpret_cast<uint4 *>(b.count + c0 + k) = make_uint4(0u, 0u, 0u, 0u);  // next step
+    }
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; ++k) {
+      if (c0 + k < cells) {
+        b.startNext[c0 + k] = st[k];
+        if (!LDS_COUNTS) b.count[c0 + k] = 0u;  // ready for the next step's counting
+      }
+    }
+  }
+  if (ldsStarts) {
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; ++k)
+      if (c0 + k < cells) ldsStarts[lds_cell(c0 + k)] = st[k];
   }
   if (threadIdx.x == 0) {
     for (int w = 1; w < kWaves; ++w) mn = min(mn, waveMin[w]), mx = max(mx, waveMax[w]);
     b.cellStats[0] = mn;
     b.cellStats[1] = mx;
     b.startNext[cells] = tot;
-    if (ldsStarts) ldsStarts[cells] = tot;
+    if (ldsStarts) ldsStarts[lds_cell(cells)] = tot;
   }
 }
 
@@ -852,8 +888,8 @@ __global__ __launch_bounds__(kSmallGridBlock) void k_scan_small(Buffers b, uint3
 // atomics.
 constexpr int kSmallItems = kSmallCountParticles / kSmallGridBlock;  // particles per thread, at most
 __global__ __launch_bounds__(kSmallGridBlock) void k_count_scan_small(Buffers b, int n, uint32_t cells) {
-  __shared__ uint32_t hist[kSmallGridCells + 1];  // counts, then cell starts [0, cells]
-  for (uint32_t c = threadIdx.x; c < cells; c += kSmallGridBlock) hist[c] = 0u;
+  __shared__ uint32_t hist[kSmallHistWords];  // counts, then cell starts [0, cells], at lds_cell(c)
+  for (uint32_t c = threadIdx.x; c < cells; c += kSmallGridBlock) hist[lds_cell(c)] = 0u;
   uint32_t key[kSmallItems], fill[kSmallItems];
 #pragma unroll
   for (int u = 0; u < kSmallItems; ++u) {  // the key loads in flight together
@@ -864,7 +900,7 @@ __global__ __launch_bounds__(kSmallGridBlock) void k_count_scan_small(Buffers b,
 #pragma unroll
   for (int u = 0; u < kSmallItems; ++u) {
     if (u * kSmallGridBlock >= n) break;
-    fill[u] = (int)threadIdx.x + u * kSmallGridBlock < n ? atomicAdd(&hist[key[u]], 1u) : 0u;  // arrival
+    fill[u] = (int)threadIdx.x + u * kSmallGridBlock < n ? atomicAdd(&hist[lds_cell(key[u])], 1u) : 0u;  // arrival
   }
   __syncthreads();
   scan_small_body<true>(b, cells, hist, hist);
@@ -874,7 +910,7 @@ __global__ __launch_bounds__(kSmallGridBlock) void k_count_scan_small(Buffers b,
     if (u * kSmallGridBlock >= n) break;
     const int i = (int)threadIdx.x + u * kSmallGridBlock;
     if (i < n) {
-      const uint32_t c0 = hist[key[u]], c1 = hist[key[u] + 1], pos = c0 + fill[u];
+      const uint32_t c0 = hist[lds_cell(key[u])], c1 = hist[lds_cell(key[u] + 1)], pos = c0 + fill[u];
       b.tmpSlot[pos] = (uint32_t)i;
       b.binRec[pos] = make_uint4((uint32_t)i, key[u], c0, c1);
     }
