@@ -424,13 +424,26 @@ __device__ __forceinline__ void block_min_max(uint32_t &mn, uint32_t &mx) {
   }
 }
 
+// The kScanItems = 4 counts of a scan thread's cells [base, base + 4), zero
+// past the last cell: one 16-byte load with no branch around it (the counter
+// array holds whole scan chunks, so the load is always in range).
+static_assert(kScanItems == 4, "a scan thread's cells are one 16-byte vector");
+__device__ __forceinline__ void load_counts4(const uint32_t *count, uint64_t base, uint32_t cells, uint32_t (&v)[4]) {
+  const uint4 q = *reinterpret_cast<const uint4 *>(count + base);
+  v[0] = base < cells ? q.x : 0u;
+  v[1] = base + 1u < cells ? q.y : 0u;
+  v[2] = base + 2u < cells ? q.z : 0u;
+  v[3] = base + 3u < cells ? q.w : 0u;
+}
+
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(Buffers b, uint32_t cells) {
   const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanItems;
   uint32_t s = 0, mx = 0, mn = 0xFFFFFFFFu;
+  uint32_t cv[kScanItems];
+  load_counts4(b.count, base, cells, cv);
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    const uint64_t c = base + k;
-    const uint32_t v = (c < cells) ? b.count[c] : 0u;
+    const uint32_t v = cv[k];
     s += v;
     mx = max(mx, v);
     if (v > 0u) mn = min(mn, v);
@@ -452,13 +465,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(Buffers b, uint32_t cell
 __global__ __launch_bounds__(kBlock) void k_scan_final(Buffers b, uint32_t cells, uint32_t n) {
   const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + (uint64_t)threadIdx.x * kScanItems;
   uint32_t v[kScanItems];
+  load_counts4(b.count, base, cells, v);
   uint32_t s = 0;
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    const uint64_t c = base + k;
-    v[k] = (c < cells) ? b.count[c] : 0u;
-    s += v[k];
-  }
+  for (int k = 0; k < kScanItems; ++k) s += v[k];
   // offset of this workgroup: the sum of the preceding workgroups' totals
   __shared__ uint32_t shOff[kBlock / 64];
   uint32_t pre = 0;
@@ -471,23 +481,34 @@ __global__ __launch_bounds__(kBlock) void k_scan_final(Buffers b, uint32_t cells
   for (int w = 0; w < kBlock / 64; ++w) blockOff += shOff[w];
   uint32_t tot;
   uint32_t run = blockOff + block_excl_scan(s, &tot);
+  uint32_t st[kScanItems];  // the cells' starts
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    st[k] = run;
+    run += v[k];
+  }
+  if (base + kScanItems <= cells) {
+    *reinterpret_cast<uint4 *>(b.startNext + base) = make_uint4(st[0], st[1], st[2], st[3]);
+    *reinterpret_cast<uint4 *>(b.count + base) = make_uint4(0u, 0u, 0u, 0u);  // ready for the next step's counting
+  }
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     const uint64_t c = base + k;
     if (c < cells) {
-      b.startNext[c] = run;
-      b.count[c] = 0u;  // ready for the next step's counting
+      if (base + kScanItems > cells) {
+        b.startNext[c] = st[k];
+        b.count[c] = 0u;
+      }
       if (b.chunkCells && v[k] > 0u) {
         // placement chunks whose first or last slot lies in this cell
-        const uint32_t first = run, last = run + v[k] - 1u, lastChunk = (n - 1u) / kPlaceChunk;
+        const uint32_t first = st[k], last = st[k] + v[k] - 1u, lastChunk = (n - 1u) / kPlaceChunk;
         for (uint32_t g = (first + kPlaceChunk - 1u) / kPlaceChunk; g * kPlaceChunk <= last; ++g)
           b.chunkCells[2 * g] = (uint32_t)c;
         for (uint32_t g = (first + 1u + kPlaceChunk - 1u) / kPlaceChunk; g <= (last + 1u) / kPlaceChunk; ++g)
           if (g >= 1u && g - 1u < lastChunk) b.chunkCells[2 * (g - 1u) + 1] = (uint32_t)c;
         if (last == n - 1u) b.chunkCells[2 * lastChunk + 1] = (uint32_t)c;
       }
-      run += v[k];
-      if (c == (uint64_t)cells - 1) b.startNext[cells] = run;
+      if (c == (uint64_t)cells - 1) b.startNext[cells] = st[k] + v[k];
     }
   }
 }

@@ -79,7 +79,7 @@ struct Buffers {
   uint32_t *tmpSlot;   // source slots bucketed by new cell
   uint4 *binRec;       // per new slot of a one-workgroup binning: source slot, key, cell start, cell end
   uint32_t *scn;       // scene of particles not yet binned
-  uint32_t *count;     // per cell, zero between steps
+  uint32_t *count;     // per cell, zero between steps (whole scan chunks; past the last cell always zero)
   uint32_t *startCur;  // per cell + 1: the last binning
   uint32_t *startNext; // per cell + 1: the binning being built
   uint32_t *blockSum;  // per scan workgroup: its cells' total

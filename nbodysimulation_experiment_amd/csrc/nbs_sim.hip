@@ -730,14 +730,14 @@ class Simulation {
   void allocCells() {
     const size_t c = totalCells_;
     const size_t nb = (c + kScanChunk - 1) / kScanChunk;
-    d_.count.alloc(c);
+    d_.count.alloc(nb * kScanChunk);  // whole scan chunks: a scan thread's four counts are one load
     d_.startCur.alloc(c + 1);
     d_.startNext.alloc(c + 1);
     d_.blockSum.alloc(std::max<size_t>(nb, 1));
     d_.cellStats.alloc(2 * std::max<size_t>(nb, 1));
     if (!d_.farCounts.p) d_.farCounts.alloc(2);
     launch_fill_u32(d_.farCounts.p, 0u, 2, stream_);
-    launch_fill_u32(d_.count.p, 0u, c, stream_);
+    launch_fill_u32(d_.count.p, 0u, nb * kScanChunk, stream_);  // the entries past the last cell stay zero
     launch_fill_u32(d_.startCur.p, 0u, c + 1, stream_);
     launch_fill_u32(d_.startNext.p, 0u, c + 1, stream_);
     NBS_HIP(hipGetLastError());
