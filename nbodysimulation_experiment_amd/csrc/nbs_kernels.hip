@@ -129,9 +129,17 @@ __device__ __forceinline__ uint32_t for_each_pair(const uint32_t *start, const D
 // of four pairs).  So the accumulation performs the same FP32 operations in
 // the same order as the one-lane-per-particle walk; only the per-pair
 // arithmetic runs in parallel across the wave.  acc0 and acc1 are the
-// accumulators' initial and final values (the same in every lane).  Each
-// chunk's position (and G value) loads are issued one chunk ahead.  Returns
-// the candidate count.
+// accumulators' initial and final values (the same in every lane).
+// The chunks are the 64-candidate pieces of the rows dy = -1, 0, 1 in order;
+// the row ranges are the wave's particle's, so the chunk sequence is
+// wave-uniform.  The position (and G value) loads of the first kWideAhead
+// chunks are issued together before the first is used (a particle of a
+// crowded scene has several chunks, and one chunk ahead left a memory latency
+// per chunk on the wave's path); later chunks are loaded one ahead.  idx(1, .)
+// of the particle's own row, which always holds the particle, serves as the
+// in-range address of the lanes past a row's end.  Returns the candidate count.
+constexpr uint32_t kWideAhead = 8;
+
 template <int K, int STAGES, bool SUB, typename Idx, typename Term>
 __device__ __forceinline__ uint32_t for_each_pair_wide(const uint32_t *start, const DevScene &sc, int cx, int cy,
                                                        const float2 *X, Idx idx, float2 xi, float h2, const float2 *G,
@@ -140,36 +148,42 @@ __device__ __forceinline__ uint32_t for_each_pair_wide(const uint32_t *start, co
   const uint32_t c = lane & 1u;
   if (lane < (uint32_t)K) lds[64 * K + lane] = SUB ? 0.0f : -0.0f;
   float acc = c ? acc1 : acc0;
-  const RowRange ra = row_range(start, sc, cx, cy, -1), rb = row_range(start, sc, cx, cy, 0),
-                 rc = row_range(start, sc, cx, cy, 1);
-  const uint32_t cand = (ra.j1 - ra.j0) + (rb.j1 - rb.j0) + (rc.j1 - rc.j0);
-  int d = 0;
-  uint32_t base = ra.j0, end = ra.j1;
-  auto skip_done_rows = [&]() {
-    while (d < 3 && base >= end) {
-      ++d;
-      if (d == 1) base = rb.j0, end = rb.j1;
-      if (d == 2) base = rc.j0, end = rc.j1;
+  RowRange rr[3];
+  row_ranges(start, sc, cx, cy, rr);
+  uint32_t j0[3], j1[3], nc[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {  // wave-uniform
+    j0[d] = (uint32_t)__builtin_amdgcn_readfirstlane((int)rr[d].j0);
+    j1[d] = (uint32_t)__builtin_amdgcn_readfirstlane((int)rr[d].j1);
+    nc[d] = (j1[d] - j0[d] + 63u) / 64u;
+  }
+  const uint32_t cand = (j1[0] - j0[0]) + (j1[1] - j0[1]) + (j1[2] - j0[2]);
+  const uint32_t nch = nc[0] + nc[1] + nc[2];
+  const uint32_t own = idx(1, j0[1]);  // an in-range slot (the own row holds the particle)
+  // chunk k: row d, first slot base, row end
+  auto chunk = [&](uint32_t k, int &d, uint32_t &base, uint32_t &end) {
+    if (k < nc[0]) {
+      d = 0;
+      base = j0[0] + 64u * k;
+    } else if (k < nc[0] + nc[1]) {
+      d = 1;
+      base = j0[1] + 64u * (k - nc[0]);
+    } else {
+      d = 2;
+      base = j0[2] + 64u * (k - nc[0] - nc[1]);
     }
+    end = j1[d];
   };
-  float2 xn = make_float2(0.0f, 0.0f), gn = make_float2(0.0f, 0.0f);
-  bool inN = false;
-  auto issue = [&]() {
-    inN = d < 3 && base + lane < end;
-    if (inN) {
-      xn = X[idx(d, base) + lane];
-      if (G) gn = G[base + lane];
-    }
+  auto load = [&](uint32_t k, float2 &x, float2 &g, bool &in, uint32_t &j) {
+    int d;
+    uint32_t base, end;
+    chunk(k, d, base, end);
+    j = base + lane;
+    in = j < end;
+    x = X[in ? idx(d, base) + lane : own];
+    if (G) g = G[in ? j : own];
   };
-  skip_done_rows();
-  issue();
-  while (d < 3) {
-    const float2 xj = xn, gj = gn;
-    const bool in = inN;
-    const uint32_t j = base + lane;
-    base += 64u;
-    skip_done_rows();
-    issue();
+  auto process = [&](float2 xj, float2 gj, bool in, uint32_t j) {
     float t[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) t[k] = 0.0f;
@@ -198,6 +212,33 @@ __device__ __forceinline__ uint32_t for_each_pair_wide(const uint32_t *start, co
       for (int u = 0; u < 4; ++u)
 #pragma unroll
         for (int st = 0; st < STAGES; ++st) acc = SUB ? acc - v[u][st] : acc + v[u][st];
+    }
+  };
+  float2 xa[kWideAhead], ga[kWideAhead];
+  bool ia[kWideAhead];
+  uint32_t ja[kWideAhead];
+#pragma unroll
+  for (uint32_t k = 0; k < kWideAhead; ++k) {
+    if (k >= nch) break;  // wave-uniform
+    ga[k] = make_float2(0.0f, 0.0f);
+    load(k, xa[k], ga[k], ia[k], ja[k]);
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < kWideAhead; ++k) {
+    if (k >= nch) break;
+    process(xa[k], ga[k], ia[k], ja[k]);
+  }
+  if (nch > kWideAhead) {  // the rest, one chunk ahead
+    float2 xn, gn = make_float2(0.0f, 0.0f);
+    bool inN;
+    uint32_t jn;
+    load(kWideAhead, xn, gn, inN, jn);
+    for (uint32_t k = kWideAhead; k < nch; ++k) {
+      const float2 xj = xn, gj = gn;
+      const bool in = inN;
+      const uint32_t j = jn;
+      if (k + 1u < nch) load(k + 1u, xn, gn, inN, jn);
+      process(xj, gj, in, j);
     }
   }
   acc0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(acc), 0));
