@@ -584,19 +584,34 @@ __device__ __forceinline__ uint32_t rank_in_cell(Slots slots, uint32_t c0, uint3
   return rank;
 }
 
-// Moves particle s of the previous slot order to slot d of the new one, key
-// k.  Every field is loaded before the first store: the destination arrays
-// are distinct from the sources, and loading first keeps the loads in flight
-// together (interleaved, each load would wait for the store before it).
-__device__ __forceinline__ void place_move(const Buffers &b, uint32_t s, uint32_t d, uint32_t k) {
-  const float2 xp = b.xp[s], x0 = b.x0[s];
-  const uint32_t id = b.id[s];
-  const float2 dn = b.densTo ? b.densFrom[s] : make_float2(0.0f, 0.0f);
-  b.x1[d] = xp;
-  b.prev[d] = x0;  // demo4.cpp:334 prevPosition = curPosition
-  b.id2[d] = id;
+// The fields a move carries: particle s of the previous slot order goes to
+// slot d of the new one with key k.  The destination arrays are distinct from
+// the sources, so every field can be loaded before the first store, which
+// keeps the loads in flight together (interleaved, each load would wait for
+// the store before it).
+struct Moved {
+  float2 xp, x0, dn;
+  uint32_t id;
+};
+__device__ __forceinline__ Moved load_moved(const Buffers &b, uint32_t s) {
+  Moved m;
+  m.xp = b.xp[s];
+  m.x0 = b.x0[s];
+  m.id = b.id[s];
+  // the carried density, or (not stored) any in-range float2 slot array: an
+  // unconditional load, which a branch would make wait at once
+  m.dn = (b.densTo ? b.densFrom : b.xp)[s];
+  return m;
+}
+__device__ __forceinline__ void store_moved(const Buffers &b, uint32_t d, uint32_t k, const Moved &m) {
+  b.x1[d] = m.xp;
+  b.prev[d] = m.x0;  // demo4.cpp:334 prevPosition = curPosition
+  b.id2[d] = m.id;
   b.keys[d] = k;
-  if (b.densTo) b.densTo[d] = dn;
+  if (b.densTo) b.densTo[d] = m.dn;
+}
+__device__ __forceinline__ void place_move(const Buffers &b, uint32_t s, uint32_t d, uint32_t k) {
+  store_moved(b, d, k, load_moved(b, s));
 }
 
 // K moves at once: every load of the K particles is issued before the first
@@ -605,24 +620,12 @@ __device__ __forceinline__ void place_move(const Buffers &b, uint32_t s, uint32_
 template <uint32_t K>
 __device__ __forceinline__ void place_moves(const Buffers &b, const uint32_t (&s)[K], const uint32_t (&d)[K],
                                             const uint32_t (&k)[K]) {
-  float2 xp[K], x0[K], dn[K];
-  uint32_t id[K];
+  Moved m[K];
 #pragma unroll
-  for (uint32_t u = 0; u < K; ++u) {
-    xp[u] = b.xp[s[u]];
-    x0[u] = b.x0[s[u]];
-    id[u] = b.id[s[u]];
-    dn[u] = b.densTo ? b.densFrom[s[u]] : make_float2(0.0f, 0.0f);
-  }
+  for (uint32_t u = 0; u < K; ++u) m[u] = load_moved(b, s[u]);
 #pragma unroll
-  for (uint32_t u = 0; u < K; ++u) {
-    if (d[u] == 0xFFFFFFFFu) continue;
-    b.x1[d[u]] = xp[u];
-    b.prev[d[u]] = x0[u];  // demo4.cpp:334 prevPosition = curPosition
-    b.id2[d[u]] = id[u];
-    b.keys[d[u]] = k[u];
-    if (b.densTo) b.densTo[d[u]] = dn[u];
-  }
+  for (uint32_t u = 0; u < K; ++u)
+    if (d[u] != 0xFFFFFFFFu) store_moved(b, d[u], k[u], m[u]);
 }
 
 // REC: the bucketing wrote a record per new slot (binRec), so each thread
@@ -646,6 +649,9 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
       c1 = b.startNext[key + 1];
     }
   }
+  // the moved fields depend only on the source slot: requested now, stored
+  // after the rank (a thread past n loads slot 0 and stores nothing)
+  const Moved mv = load_moved(b, live ? s : 0u);
   if (p == p0) span[0] = c0;
   if (p == min(p0 + kBlock, n) - 1) span[1] = c1;
   __syncthreads();
@@ -657,7 +663,7 @@ __global__ __launch_bounds__(kBlock) void k_rank_reorder(Buffers b, int n) {
   if (!live) return;
   const uint32_t rank = tiled ? rank_in_cell([&](uint32_t q) { return tileSlots[q - lo]; }, c0, c1, s)
                               : rank_in_cell([&](uint32_t q) { return b.tmpSlot[q]; }, c0, c1, s);
-  place_move(b, s, c0 + rank, key);
+  store_moved(b, c0 + rank, key, mv);
 }
 
 // ---------------------------------------------------------------------------
