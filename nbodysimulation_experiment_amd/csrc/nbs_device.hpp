@@ -441,27 +441,39 @@ __device__ __forceinline__ void plan_tile(const uint32_t *start, const DevScene 
   t.use = (CAP > 0 && total <= (uint32_t)CAP) ? 1 : 0;
 }
 
-// Cooperative, coalesced copy of the planned ranges of src into the tile.
-// Each thread keeps four loads in flight.  Tile entry q of range d is slot
-// q + (rowStart[d] - off[d]), formed in 32 bits (the difference wraps).
-// Entries past the end are clamped to the last one, which is then loaded and
-// stored more than once with the same value.
-__device__ __forceinline__ void stage(const TilePlan &t, const float2 *__restrict__ src, float2 *dst) {
+// Cooperative, coalesced copy of the planned ranges of N source arrays into N
+// tiles.  Each thread keeps four loads per array in flight, all arrays'
+// together.  Tile entry q of range d is slot q + (rowStart[d] - off[d]),
+// formed in 32 bits (the difference wraps).  Entries past the end are clamped
+// to the last one, which is then loaded and stored more than once with the
+// same value.
+template <int N>
+__device__ __forceinline__ void stage_arrays(const TilePlan &t, const float2 *const (&src)[N],
+                                             float2 *const (&dst)[N]) {
   constexpr uint32_t kU = 4;
   const uint32_t total = t.off[3], off1 = t.off[1], off2 = t.off[2];
   const uint32_t b0 = t.rowStart[0] - t.off[0], b1 = t.rowStart[1] - off1, b2 = t.rowStart[2] - off2;
   for (uint32_t q0 = threadIdx.x; q0 < total; q0 += kU * blockDim.x) {
-    float2 v[kU];
+    float2 v[N][kU];
     uint32_t at[kU];
 #pragma unroll
     for (uint32_t u = 0; u < kU; ++u) {
       at[u] = min(q0 + u * blockDim.x, total - 1u);
       const uint32_t b = at[u] >= off2 ? b2 : (at[u] >= off1 ? b1 : b0);
-      v[u] = src[(uint32_t)(at[u] + b)];
+#pragma unroll
+      for (int a = 0; a < N; ++a) v[a][u] = src[a][(uint32_t)(at[u] + b)];
     }
 #pragma unroll
-    for (uint32_t u = 0; u < kU; ++u) dst[at[u]] = v[u];
+    for (uint32_t u = 0; u < kU; ++u)
+#pragma unroll
+      for (int a = 0; a < N; ++a) dst[a][at[u]] = v[a][u];
   }
+}
+
+__device__ __forceinline__ void stage(const TilePlan &t, const float2 *src, float2 *dst) {
+  const float2 *const s[1] = {src};
+  float2 *const d[1] = {dst};
+  stage_arrays<1>(t, s, d);
 }
 
 // Index of slot j of row range d in the neighbour data source.

@@ -158,22 +158,27 @@ struct PartInfo {
   float fr, fn;  // normalised density, near density
 };
 
+// A particle's own loads for part_info, requested before the tile is staged
+// (the staging's waits then cover them) and turned into PartInfo after.
+struct PartRaw {
+  uint32_t key;
+  float2 x, pv, d;
+};
+__device__ __forceinline__ PartRaw part_raw(const Buffers &b, int i) { return {b.keys[i], b.x1[i], b.prev[i], b.dens[i]}; }
+
 template <bool MULTI>
 __device__ __forceinline__ PartInfo part_info(const Buffers &b, const DevParams &p, const DevScene &s0,
-                                              const float *norm, int i) {
+                                              const float *norm, const PartRaw &r) {
   PartInfo q;
-  const uint32_t key = b.keys[i];
-  q.sc = scene_of_slot<MULTI>(b, s0, key);
-  const uint32_t local = key - q.sc.cellBase;
+  q.sc = scene_of_slot<MULTI>(b, s0, r.key);
+  const uint32_t local = r.key - q.sc.cellBase;
   q.cy = (int)(local / (uint32_t)q.sc.nx);
   q.cx = (int)local - q.cy * q.sc.nx;
-  q.xi = b.x1[i];
-  const float2 pv = b.prev[i];
-  q.vix = (q.xi.x - pv.x) * p.invDt;
-  q.viy = (q.xi.y - pv.y) * p.invDt;
-  const float2 d = b.dens[i];
-  q.fr = (d.x - norm[1]) * norm[2];
-  q.fn = (d.y - norm[3]) * norm[4];
+  q.xi = r.x;
+  q.vix = (q.xi.x - r.pv.x) * p.invDt;
+  q.viy = (q.xi.y - r.pv.y) * p.invDt;
+  q.fr = (r.d.x - norm[1]) * norm[2];
+  q.fn = (r.d.y - norm[3]) * norm[4];
   return q;
 }
 
@@ -238,9 +243,9 @@ __device__ __forceinline__ PairSrc stage_pairs(const Buffers &b, const DevScene 
   if (src.tiled) {
     float2 *tx = reinterpret_cast<float2 *>(pool);
     float2 *tp = tx + kTileCap, *td = tp + kTileCap;
-    stage(plan, b.x1, tx);
-    stage(plan, b.prev, tp);
-    stage(plan, b.dens, td);
+    const float2 *const from[3] = {b.x1, b.prev, b.dens};
+    float2 *const to[3] = {tx, tp, td};
+    stage_arrays<3>(plan, from, to);  // the three arrays' loads in flight together
     src.x = tx;
     src.pv = tp;
     src.d = td;
@@ -468,13 +473,14 @@ __global__ __launch_bounds__(64, 3) void k_sur_forward(Buffers b, DevParams p, D
   const float *norm = prm + kOffNorm;
   const W1Operands w1 = load_w1(m.w1b);
   const bf16x8 sel[2] = {selection(0), selection(1)};
+  const PartRaw raw = part_raw(b, min(me, n - 1));  // a lane past n reads the last particle, unused
   const PairSrc src = stage_pairs<MULTI>(b, s0, base, n, pool, plan);
 
   // 1. pair layer: H_i = sum_j relu(W1 f_ij), each term rounded to BF16
   PartInfo q{};
   Cursor cu{0u, 0u, 0u, 0, true, 0u, 0u};
   if (me < n) {
-    q = part_info<MULTI>(b, p, s0, norm, me);
+    q = part_info<MULTI>(b, p, s0, norm, raw);
     cu.done = false;
   }
   floatx16 h00 = zero16(), h01 = zero16(), h10 = zero16(), h11 = zero16();  // tile, hidden half
@@ -892,6 +898,7 @@ __global__ __launch_bounds__(64) void k_sur_back_pairs(Buffers b, DevParams p, D
   for (int base = (int)blockIdx.x * kWaveRows; base < n; base += (int)gridDim.x * kWaveRows) {
     const int me = base + lane;
     __syncthreads();  // the previous group's tile is no longer read
+    const PartRaw raw = part_raw(b, min(me, n - 1));  // a lane past n reads the last particle, unused
     const PairSrc src = stage_pairs<MULTI>(b, s0, base, n, pool, plan);
     // dH of the rows this lane's accumulator registers hold
     floatx16 d00, d01, d10, d11;  // tile, hidden half
@@ -906,7 +913,7 @@ __global__ __launch_bounds__(64) void k_sur_back_pairs(Buffers b, DevParams p, D
     PartInfo q{};
     Cursor cu{0u, 0u, 0u, 0, true, 0u, 0u};
     if (me < n) {
-      q = part_info<MULTI>(b, p, s0, norm, me);
+      q = part_info<MULTI>(b, p, s0, norm, raw);
       cu.done = false;
     }
     for (;;) {
