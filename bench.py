@@ -432,13 +432,16 @@ def main():
         vpath = os.path.join(ROOT, "profiles", "valu_issue.json")
         if os.path.exists(vpath):
             v = json.load(open(vpath))
-            ins = v.get("valu_insts_per_launch", {}).get(dom)
-            if v.get("workload") == args.workload and v.get("steps") == K and ins:
+            # the counters of this command's window: the profiled step count,
+            # or a window of another step count from the 500-step counter run
+            win = v if v.get("steps") == K else v.get("windows", {}).get(str(K))
+            ins = (win or {}).get("valu_insts_per_launch", {}).get(dom)
+            if v.get("workload") == args.workload and ins:
                 g = ins / (avg_ms * 1e-3) / 1e9
                 valu = {"bound": "valu", "achieved": round(g, 2), "peak": VALU_PEAK_GINST,
                         "unit": "G wave64 VALU instructions/s", "frac": round(g / VALU_PEAK_GINST, 5),
-                        "valu_insts_per_launch": ins, "issue_frac_under_counters": v["kernels"].get(dom),
-                        "source": "profiles/valu_issue.json (rocprofv3 SQ_INSTS_VALU of this command)"}
+                        "valu_insts_per_launch": ins, "issue_frac_under_counters": win["kernels"].get(dom),
+                        "source": "profiles/valu_issue.json (rocprofv3 SQ_INSTS_VALU of this command's window)"}
         primary = valu if valu is not None and valu["frac"] > hbm["frac"] else hbm
         roofline = dict(primary)
         roofline.update({"traffic": traffic, "kernel": dom, "kernel_avg_ms": round(avg_ms, 5),

@@ -24,6 +24,7 @@ import argparse
 import collections
 import csv
 import json
+import os
 import re
 import sys
 
@@ -189,6 +190,7 @@ def step_ranges(path: str, bounds: list[int]) -> list[dict]:
         if c.get("SQ_INSTS_VALU") and c.get("GRBM_GUI_ACTIVE"):
             row["valu_issue"] = round(c["SQ_INSTS_VALU"] / (CUS * c["GRBM_GUI_ACTIVE"] / 8.0), 3)
             row["valu_instr_per_launch_M"] = round(c["SQ_INSTS_VALU"] / launches[(i, n)] / 1e6, 2)
+            row["valu_instr_per_launch"] = round(c["SQ_INSTS_VALU"] / launches[(i, n)])
             row["cycles_per_launch_k"] = round(c["GRBM_GUI_ACTIVE"] / 8.0 / launches[(i, n)] / 1e3, 1)
         if c.get("SQ_THREAD_CYCLES_VALU") and c.get("SQ_INSTS_VALU"):
             row["active_lanes"] = round(c["SQ_THREAD_CYCLES_VALU"] / (64.0 * c["SQ_INSTS_VALU"]), 3)
@@ -196,6 +198,17 @@ def step_ranges(path: str, bounds: list[int]) -> list[dict]:
             row["wave_cycles_per_busy_cycle"] = round(c["SQ_WAVE_CYCLES"] / c["SQ_BUSY_CYCLES"], 2)
         out.append(row)
     return out
+
+
+def valu_window(path: str, first: int, end: int, steps: int) -> dict:
+    """The pair kernels' VALU issue and wave64 VALU instructions per launch over
+    the steps [first, end) of a counter run: the timed window of a bench.py
+    command of ``steps`` steps, for bench.py's roofline at that step count."""
+    rows = step_ranges(path, [first, end])
+    return {"steps": steps, "steps_counted": [first, end],
+            "kernels": {r["kernel"]: r["valu_issue"] for r in rows if "valu_issue" in r},
+            "valu_insts_per_launch": {r["kernel"]: r["valu_instr_per_launch"] for r in rows
+                                      if "valu_instr_per_launch" in r}}
 
 
 def main(argv=None):
@@ -210,8 +223,16 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=0, help="timed steps of the profiled bench.py command")
     a = ap.parse_args(argv)
     if a.mode == "steps":
-        for row in step_ranges(a.path, [int(x) for x in a.ranges.split(",")]):
+        bounds = [int(x) for x in a.ranges.split(",")]
+        for row in step_ranges(a.path, bounds):
             print(json.dumps(row))
+        if a.json and a.steps:
+            # the whole range as the window of a --steps run, beside the
+            # file's other windows
+            d = json.load(open(a.json)) if os.path.exists(a.json) else {"workload": a.workload}
+            d.setdefault("windows", {})[str(a.steps)] = valu_window(a.path, bounds[0], bounds[-1], a.steps)
+            with open(a.json, "w") as fh:
+                fh.write(json.dumps(d, indent=1) + "\n")
         return
     if a.mode == "traffic":
         t = traffic(a.path, a.path2, a.copy_bytes, a.workload)
@@ -238,8 +259,13 @@ def main(argv=None):
         return
     c = counters(a.path)
     if a.json:
+        d = valu_issue(c, a.workload, a.steps)
+        if os.path.exists(a.json):  # keep the windows of other step counts
+            old = json.load(open(a.json))
+            if old.get("workload") == a.workload and "windows" in old:
+                d["windows"] = old["windows"]
         with open(a.json, "w") as fh:
-            fh.write(json.dumps(valu_issue(c, a.workload, a.steps), indent=1) + "\n")
+            fh.write(json.dumps(d, indent=1) + "\n")
     if a.copy_bytes and "copy" in c:
         for key in ("FETCH_SIZE", "WRITE_SIZE"):
             if key in c["copy"]:

@@ -3,6 +3,7 @@ counter files (no GPU)."""
 from __future__ import annotations
 
 import csv
+import json
 
 from nbodysimulation_experiment_amd import profiling
 
@@ -40,6 +41,25 @@ def test_step_ranges_sum_per_range_and_kernel(tmp_path):
     assert r["valu_issue"] == round(3000.0 / (256 * 160.0 / 8.0), 3)
     assert r["active_lanes"] == 0.5
     assert rows[((2, 4), "delta")]["launches"] == 2
+
+
+def test_valu_window_merges_beside_the_profiled_window(tmp_path):
+    launches = []
+    for step in range(6):
+        c = {"SQ_INSTS_VALU": 1000.0 * (step + 1), "GRBM_GUI_ACTIVE": 8.0 * 10.0}
+        launches += [("viscosity_predict", c), ("density", c), ("delta", c)]
+    p = tmp_path / "c.csv"
+    _write(p, launches)
+    j = tmp_path / "valu.json"
+    j.write_text('{"workload": "w", "steps": 20, "kernels": {"delta": 1.0}, "valu_insts_per_launch": {"delta": 7}}')
+    profiling.main(["steps", str(p), "--ranges", "2,4,6", "--json", str(j), "--steps", "4", "--workload", "w"])
+    d = json.loads(j.read_text())
+    assert d["steps"] == 20 and d["valu_insts_per_launch"]["delta"] == 7  # the profiled window is kept
+    w = d["windows"]["4"]
+    assert w["steps_counted"] == [2, 6]
+    # steps 2..5 issue 3000..6000 instructions: 4500 per launch on average
+    assert w["valu_insts_per_launch"]["delta"] == 4500
+    assert w["kernels"]["density"] == round(18000.0 / (256 * 4 * 80.0 / 8.0), 3)
 
 
 def test_short_names_keep_tile_variants():
