@@ -118,16 +118,18 @@ void launch_unbin(const Buffers &b, int nBinned, hipStream_t stream);
 void launch_visc_predict(const Buffers &b, const DevParams &p, const DevScene &s0, int n, int nBinned, int lists,
                          int cap, uint32_t cells, hipStream_t stream);
 // Whether the cell counting of a step runs in the grid phase's one-workgroup
-// scan (LDS histogram) instead of after the viscosity kernel: wide viscosity
-// (viscCap == kWideCap), at most kSmallGridCells cells and
-// kSmallCountParticles particles.
+// scan (LDS histogram), which then also buckets the particles, instead of
+// after the viscosity kernel: wide viscosity (viscCap == kWideCap), at most
+// kSmallGridCells cells and kSmallCountParticles particles.
 bool count_in_grid(int n, uint32_t cells, int viscCap);
 // n: particles (with b.chunkCells set, the scan also fills the chunk table).
 void launch_scan(const Buffers &b, uint32_t cells, int n, hipStream_t stream);
 void launch_scatter_reorder(const Buffers &b, int n, hipStream_t stream);
 // The grid phase: the cell scan (one workgroup for small grids, else
-// launch_scan), then launch_scatter_reorder, or with place = true the cell scan
-// and k_place (b.farList etc. set, see place_binning).  Returns the number of
+// launch_scan), then launch_scatter_reorder; with count_in_grid the one
+// workgroup counts, scans and buckets, and only the rank pass follows (from
+// its records, b.binRec); with place = true the cell scan and k_place
+// (b.farList etc. set, see place_binning).  Returns the number of
 // (min, max) occupied-cell pairs it leaves in cellStats.
 uint32_t launch_grid(const Buffers &b, const DevScene &s0, uint32_t cells, int n, int viscCap, bool place,
                      hipStream_t stream);
