@@ -31,10 +31,16 @@ __device__ __forceinline__ int scene_of_key(const uint32_t *base, int n, uint32_
   return lo;
 }
 
+// Scene of cell `key`: one load from the per-cell table when the launch has
+// one, else a binary search of the scenes' first cells.
+__device__ __forceinline__ int scene_index(const Buffers &b, uint32_t key) {
+  return b.cellScene ? (int)b.cellScene[key] : scene_of_key(b.sceneBase, b.nScenes, key);
+}
+
 template <bool MULTI>
 __device__ __forceinline__ DevScene scene_of_slot(const Buffers &b, const DevScene &s0, uint32_t key) {
   if (!MULTI) return s0;
-  return b.scenes[scene_of_key(b.sceneBase, b.nScenes, key)];
+  return b.scenes[scene_index(b, key)];
 }
 
 // Correctly rounded FP32 square root and reciprocal of the pair kernels'

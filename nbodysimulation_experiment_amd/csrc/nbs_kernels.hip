@@ -406,7 +406,7 @@ __global__ __launch_bounds__(kBlock) void k_visc_predict(Buffers b, DevParams p,
 __global__ __launch_bounds__(kBlock) void k_unbin(Buffers b, int nBinned) {
   const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
   if (i >= nBinned) return;
-  b.scn[i] = (uint32_t)scene_of_key(b.sceneBase, b.nScenes, b.keys[i]);
+  b.scn[i] = (uint32_t)scene_index(b, b.keys[i]);
   b.acc[i] = make_float2(0.0f, 0.0f);
 }
 
@@ -748,7 +748,7 @@ template <bool MULTI, uint32_t kU, typename F>
 __device__ __forceinline__ void for_each_place_source(const Buffers &b, const PlaceWindows &pw0, uint32_t cf,
                                                       uint32_t cl, uint32_t farN, F &&f) {
   if (MULTI) {
-    const int sFirst = scene_of_key(b.sceneBase, b.nScenes, cf), sLast = scene_of_key(b.sceneBase, b.nScenes, cl);
+    const int sFirst = scene_index(b, cf), sLast = scene_index(b, cl);
     for (int si = sFirst; si <= sLast; ++si)
       for_each_window_source<kU>(b, place_windows(b, b.scenes[si], cf, cl), cf, cl, f);
   } else {
@@ -825,7 +825,7 @@ __global__ __launch_bounds__(kBlock) void k_place(Buffers b, DevScene s0, int n,
     const uint32_t c0 = b.startNext[k], c1 = b.startNext[k + 1];
     if (c1 <= g0 || c0 >= g1) return;
     uint32_t rank = 0;
-    const int si = MULTI ? scene_of_key(b.sceneBase, b.nScenes, k) : 0;
+    const int si = MULTI ? scene_index(b, k) : 0;
     const DevScene sc = MULTI ? b.scenes[si] : s0;
     const int64_t base = sc.cellBase, last = base + (int64_t)sc.nx * sc.ny - 1;
     for (int dy = -1; dy <= 1; ++dy) {
@@ -1310,7 +1310,7 @@ __global__ __launch_bounds__(kBlock) void k_gather_state(Buffers b, DevScene s0,
   if (key) key[id] = b.keys[i];
   if (scene) {
     int s = 0;
-    if (b.nScenes > 1) s = (i < nBinned) ? scene_of_key(b.sceneBase, b.nScenes, b.keys[i]) : (int)b.scn[i];
+    if (b.nScenes > 1) s = (i < nBinned) ? scene_index(b, b.keys[i]) : (int)b.scn[i];
     scene[id] = s;
   }
 }

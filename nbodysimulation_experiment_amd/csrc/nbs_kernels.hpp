@@ -87,6 +87,7 @@ struct Buffers {
   uint32_t *cellStats; // per scan workgroup: min, max occupied-cell count
   const DevScene *scenes;
   const uint32_t *sceneBase;  // cellBase of each scene, ascending
+  const uint16_t *cellScene;  // scene of each cell (multi-scene steps), or null: search sceneBase
   int nScenes;
   const DevBody *bodies;
   const float2 *verts;

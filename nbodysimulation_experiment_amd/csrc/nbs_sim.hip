@@ -728,6 +728,7 @@ class Simulation {
     destroyGraph();
   }
   void allocCells() {
+    cellSceneCells_ = 0;  // the per-cell scene table no longer matches (rebuilt by syncBodies)
     const size_t c = totalCells_;
     const size_t nb = (c + kScanChunk - 1) / kScanChunk;
     d_.count.alloc(nb * kScanChunk);  // whole scan chunks: a scan thread's four counts are one load
@@ -773,6 +774,8 @@ class Simulation {
     buf_.blockSum = d_.blockSum.p;
     buf_.candStats = d_.candStats.p; buf_.cellStats = d_.cellStats.p;
     buf_.scenes = devScenes_.p; buf_.sceneBase = devSceneBase_.p; buf_.nScenes = (int)scenes_.size();
+    // the per-cell scene table once syncBodies has built it for these cells
+    buf_.cellScene = (scenes_.size() > 1 && cellSceneCells_ == totalCells_) ? devCellScene_.p : nullptr;
     buf_.bodies = devBodies_.p; buf_.verts = devVerts_.p;
   }
   // Cull box of a circle (collide_wave): positions x < cx - reach or
@@ -843,6 +846,21 @@ class Simulation {
     NBS_HIP(hipMemcpyAsync(devScenes_.p, hostScenes_.data(), hostScenes_.size() * sizeof(DevScene),
                            hipMemcpyHostToDevice, stream_));
     NBS_HIP(hipMemcpyAsync(devSceneBase_.p, bases.data(), bases.size() * 4, hipMemcpyHostToDevice, stream_));
+    // scene of every cell: the multi-scene kernels' scene lookup in one load
+    // (up to 65,536 scenes; beyond that they search the scenes' first cells)
+    std::vector<uint16_t> cellScene;
+    if (scenes_.size() > 1 && scenes_.size() <= 65536) {
+      cellScene.resize(totalCells_);
+      for (size_t si = 0; si < scenes_.size(); ++si) {
+        const HostScene &hs = scenes_[si];
+        std::fill(cellScene.begin() + hs.cellBase, cellScene.begin() + hs.cellBase + (size_t)hs.nx * hs.ny,
+                  (uint16_t)si);
+      }
+      if (devCellScene_.n < cellScene.size()) devCellScene_.alloc(cellScene.size());
+      NBS_HIP(hipMemcpyAsync(devCellScene_.p, cellScene.data(), cellScene.size() * sizeof(uint16_t),
+                             hipMemcpyHostToDevice, stream_));
+    }
+    cellSceneCells_ = cellScene.empty() ? 0u : totalCells_;
     NBS_HIP(hipStreamSynchronize(stream_));
     bodiesDirty_ = scenesDirty_ = false;
     rebindBuffers();
@@ -1400,6 +1418,8 @@ class Simulation {
   Buffers buf_{};
   DevBuf<DevScene> devScenes_;
   DevBuf<uint32_t> devSceneBase_;
+  DevBuf<uint16_t> devCellScene_;  // scene of each cell (multi-scene)
+  uint32_t cellSceneCells_ = 0;     // cells the table covers; 0 when stale or absent
   DevBuf<DevBody> devBodies_;
   DevBuf<float2> devVerts_;
   DevBuf<float2> stX_, stPrev_, stV_, stDens_, stPres_;
