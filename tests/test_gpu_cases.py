@@ -220,6 +220,42 @@ def test_many_circles_and_mixed_body_lists_bit_exact(caps):
         assert_bit_exact(sim.get_state(), ref.state(), f"{len(specs)} scene(s), caps {caps}: ")
 
 
+@pytest.mark.parametrize("caps", [(1024, 1024, 1024), (-1, -1, -1)])
+def test_scene_added_mid_run_bit_exact(caps):
+    """A scene added after steps have run grows the cell tables: the per-cell
+    scene table of the multi-scene kernels is rebuilt (the old one no longer
+    covers the grid), and the next step starts without neighbour lists, as in
+    the oracle.  Two scenes over 19,200 cells (placement binning with the
+    narrow kernels), then a third; state and grid bit-exact at every mark."""
+    a = SceneSpec(120.0, 100.0, 1.0, volume_positions((-20.0, -10.0), 40, 30, 0.5, 1.0, seed=51),
+                  planes=wall_planes(120.0, 100.0))
+    b = SceneSpec(90.0, 80.0, 1.0, volume_positions((-8.0, -5.0), 24, 20, 0.5, 1.0, seed=52),
+                  planes=wall_planes(90.0, 80.0))
+    c = SceneSpec(60.0, 50.0, 1.0, volume_positions((-4.0, 0.0), 20, 16, 0.5, 1.0, seed=53),
+                  planes=wall_planes(60.0, 50.0))
+    sim, ref = gpu_for([a, b]), oracle_for([a, b])
+    sim.set_tuning(autotune=False, tile_cap=caps, wide_max_particles=0)
+
+    def check(what):
+        assert_bit_exact(sim.get_state(), ref.state(), what)
+        ids, start = sim.get_grid()
+        oids, ostart = ref.grid()
+        np.testing.assert_array_equal(start, ostart)
+        np.testing.assert_array_equal(ids, oids)
+
+    sim.Update(DT, 8)
+    ref.step(DT, 8)
+    check(f"two scenes, caps {caps}: ")
+    assert sim.add_scene_spec(c) == 2 and ref.add_scene_spec(c) == 2
+    assert sim.total_cells == 12000 + 7200 + 3000
+    sim.Update(DT, 8)
+    ref.step(DT, 8)
+    check(f"third scene added, caps {caps}: ")
+    sim.Update(DT, 8)
+    ref.step(DT, 8)
+    check(f"eight steps on, caps {caps}: ")
+
+
 @pytest.mark.parametrize("caps", [(1024, 1024, 1024), (0, 0, 2048), (1024, -1, 1024)])
 def test_dense_cells_bit_exact(caps):
     """Particles at 0.12h spacing put about 70 in a cell: rows of 200+
