@@ -307,19 +307,33 @@ __device__ __forceinline__ WaveBox wave_box(float x, float y, bool live) {
 
 // Bodies among [c0, c0 + cnt) (cnt <= 64) that may move a position inside
 // `box`: bit t for body c0 + t.  A circle is excluded when the box lies
-// outside its cull box; every other body is included.
-__device__ __forceinline__ uint64_t body_candidates(const DevBody *bodies, int c0, int cnt, const WaveBox &box) {
+// outside its cull box.  A plane is excluded when its projection, computed as
+// solve_plane computes it, exceeds R at the box corner on the plane's side
+// (the lowest x for nx >= 0, else the highest, and the same for y): each step
+// of that computation (x - nx d, times nx, the sum) is a rounded subtraction,
+// product or sum, monotone in the position for a fixed normal, so the
+// corner's projection is at most any live position's, and every live
+// position's test proj <= R fails as well (a NaN anywhere keeps the plane).
+// Every other body is included.
+__device__ __forceinline__ uint64_t body_candidates(const DevBody *bodies, int c0, int cnt, const WaveBox &box,
+                                                    float R) {
   const int t = threadIdx.x & 63;
   bool may = false;
   if (t < cnt) {
-    // the type and the whole cull box (16 bytes at offset 32 of the 48-byte
-    // record) are loaded together and tested without short-circuiting, so the
-    // test costs one memory round trip
+    // the type, the parameters and the whole cull box (the record's 48 bytes
+    // as three 16-byte loads) are loaded together and tested without
+    // short-circuiting, so the test costs one memory round trip
     const DevBody &bd = bodies[c0 + t];
     const int type = bd.type;
+    const float4 a = *reinterpret_cast<const float4 *>(&bd.a0);
     const int4 cl = *reinterpret_cast<const int4 *>(bd.cull);
     const bool outside = (cl.x > box.x1) | (cl.z < box.x0) | (cl.y > box.y1) | (cl.w < box.y0);
-    may = (type != BODY_CIRCLE) | !outside;
+    const float cx = __int_as_float(float_order(a.x >= 0.0f ? box.x0 : box.x1));
+    const float cy = __int_as_float(float_order(a.y >= 0.0f ? box.y0 : box.y1));
+    const float ppx = a.x * a.z, ppy = a.y * a.z;  // solve_plane's operations, in its order
+    const float proj = (cx - ppx) * a.x + (cy - ppy) * a.y;
+    const bool beyond = proj > R;  // false for NaN
+    may = ((type != BODY_CIRCLE) | !outside) & ((type != BODY_PLANE) | !beyond);
   }
   return __ballot(may);
 }
@@ -330,7 +344,8 @@ __device__ __forceinline__ uint64_t body_candidates(const DevBody *bodies, int c
 // bodies that may touch it are applied, in body order.  A circle is skipped
 // when the box lies outside its cull box: every live position is then farther
 // than both * (1 + 2^-8) from the centre along one axis, so its FP32 test
-// d2 <= both^2 fails and the circle would leave it unchanged.  After a body
+// d2 <= both^2 fails and the circle would leave it unchanged.  A plane is
+// skipped when its test fails at the box's nearest corner (body_candidates).  After a body
 // moves a live position the box and the remaining candidates are recomputed.
 // Lanes that are not live compute on whatever x, y hold and must not store
 // the result.  All 64 lanes of the wave must be active.
@@ -353,7 +368,7 @@ __device__ __forceinline__ void collide_wave(const Buffers &b, const DevParams &
   WaveBox box = wave_box(x, y, live);
   for (int c0 = k0; c0 < k1; c0 += 64) {
     const int cnt = min(64, k1 - c0);
-    uint64_t cand = body_candidates(b.bodies, c0, cnt, box);
+    uint64_t cand = body_candidates(b.bodies, c0, cnt, box, p.R);
     while (cand) {
       const int t = __builtin_amdgcn_readfirstlane(__builtin_ctzll(cand));
       cand &= cand - 1u;
@@ -369,7 +384,7 @@ __device__ __forceinline__ void collide_wave(const Buffers &b, const DevParams &
       const bool moved = (__float_as_uint(x) != __float_as_uint(px)) || (__float_as_uint(y) != __float_as_uint(py));
       if (__ballot(live && moved)) {
         box = wave_box(x, y, live);
-        cand = body_candidates(b.bodies, c0, cnt, box) & ~((2ull << t) - 1u);  // bodies after t
+        cand = body_candidates(b.bodies, c0, cnt, box, p.R) & ~((2ull << t) - 1u);  // bodies after t
       }
     }
   }

@@ -51,8 +51,8 @@ struct DevBody {
   // the circle's collision test (x0, y0, x1, y1); full range for other bodies.
   int cull[4];
 };
-// body_candidates reads cull as one 16-byte load
-static_assert(sizeof(DevBody) == 48 && offsetof(DevBody, cull) == 32, "DevBody layout");
+// body_candidates reads a0..a3 and cull as one 16-byte load each
+static_assert(sizeof(DevBody) == 48 && offsetof(DevBody, a0) == 16 && offsetof(DevBody, cull) == 32, "DevBody layout");
 
 // Order-preserving integer image of a float: for non-NaN a, b,
 // a < b implies float_order(a) < float_order(b) (and -0 maps below +0).
