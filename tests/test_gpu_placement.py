@@ -49,3 +49,15 @@ def test_cells_of_thousands_bit_exact():
     spec = SceneSpec(200.0, 200.0, 1.0, pos, planes=wall_planes(200.0, 200.0))
     st = _run(spec, SPHParams(k=0.0, k_near=0.0, sigma=0.0, beta=0.0), (0.0, -9.8), [3, 12])
     assert st["maxCellParticleCount"] >= 5000
+
+
+def test_cell_count_not_a_multiple_of_four_bit_exact():
+    # 131 x 127 = 16,637 cells: above the one-workgroup scan, and the last scan
+    # thread's four cells end past the grid (its counts are zeroed there and
+    # its starts stored one by one)
+    pos = volume_positions((-20.0, -15.0), 90, 80, 0.5, 1.0, seed=33)
+    spec = SceneSpec(131.0, 127.0, 1.0, pos, planes=wall_planes(131.0, 127.0))
+    sim = gpu_for([spec], SPHParams(), (0.0, -9.8))
+    assert sim.total_cells == 16637
+    sim.close()
+    _run(spec, SPHParams(), (0.0, -9.8), [4, 12])
